@@ -1,0 +1,363 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by the cpu_baseline leg of bench.py, always as the checker or the timed CPU
+baseline.  The product package (muspectre_amd) never imports it.
+
+Array conventions follow the reference (paths relative to its root):
+fields are (N, nc) C-order numpy arrays, i.e. AoS with the tensor component
+i + dim*j of A(i,j) (src/common/field_typed.hh:119); spectra are
+(Kpts, nc) complex128 arrays over the Hermitian grid
+(src/fft/fftw_engine.cc:40-77).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FINITE_STRAIN, SMALL_STRAIN = 0, 1
+PROJ_SMALL, PROJ_FINITE_FAST, PROJ_FINITE = 0, 1, 2
+LE1, LE2, LE3, LE4, HEP1 = 1, 2, 3, 4, 5
+
+ERRORS = {1: "ConvergenceError", 2: "SolverError", 3: "ProjectionError",
+          4: "RuntimeError", 5: "MemoryError"}
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("{}: {}".format(ERRORS.get(code, code), msg))
+        self.code = code
+        self.kind = ERRORS.get(code, str(code))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        vp = ctypes.c_void_p
+        L.mo_last_error.restype = ctypes.c_char_p
+        L.mo_cell_create.argtypes = [ctypes.c_int, ip, dp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(vp)]
+        L.mo_cell_destroy.argtypes = [vp]
+        L.mo_cell_add_material.argtypes = [vp, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double]
+        L.mo_material_add_pixel.argtypes = [vp, ctypes.c_int, ctypes.c_long, dp]
+        for name in ("mo_cell_initialise", "mo_cell_evaluate_stress_tangent",
+                     "mo_cell_evaluate_stress", "mo_cell_save_history"):
+            getattr(L, name).argtypes = [vp]
+        for name in ("mo_cell_strain", "mo_cell_stress", "mo_cell_tangent"):
+            getattr(L, name).argtypes = [vp]
+            getattr(L, name).restype = dp
+        L.mo_cell_directional_stiffness.argtypes = [vp, dp, dp]
+        L.mo_cell_tangent_product.argtypes = [vp, dp, dp]
+        L.mo_cell_apply_projection.argtypes = [vp, dp]
+        L.mo_cell_get_internal.argtypes = [vp, ctypes.c_char_p, ctypes.c_int,
+                                           dp, ip]
+        L.mo_cg_solve.argtypes = [vp, ctypes.c_double, ctypes.c_int, dp, dp,
+                                  ctypes.POINTER(ctypes.c_long)]
+        L.mo_cg_fixed_iterations.argtypes = [vp, dp, dp, ctypes.c_int]
+        drv = [vp, ctypes.c_int, dp, ctypes.c_double, ctypes.c_int,
+               ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_long),
+               ctypes.c_void_p, dp, dp]
+        L.mo_newton_cg.argtypes = drv
+        L.mo_de_geus.argtypes = drv
+        L.mo_fft_r2c.argtypes = [ctypes.c_int, ip, ctypes.c_int, dp, dp]
+        L.mo_fft_c2r.argtypes = [ctypes.c_int, ip, ctypes.c_int, dp, dp]
+        L.mo_fft_freqs.argtypes = [ctypes.c_int, ctypes.c_double, dp]
+        L.mo_projection_operator.argtypes = [ctypes.c_int, ctypes.c_int, ip, dp,
+                                             ctypes.c_int, dp]
+        L.mo_apply_projection.argtypes = [ctypes.c_int, ctypes.c_int, ip, dp,
+                                          ctypes.c_int, dp]
+        for name in ("mo_eig_sym3", "mo_eig_sym2", "mo_expm_eig"):
+            getattr(L, name).argtypes = [dp, dp]
+        L.mo_eigsys_sym3.argtypes = [dp, dp, dp]
+        for name in ("mo_logm", "mo_expm", "mo_logm_sylvester", "mo_inverse"):
+            getattr(L, name).argtypes = [ctypes.c_int, dp, dp]
+        L.mo_le1_eval.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, dp, dp, dp]
+        L.mo_hep1_eval_kirchhoff.argtypes = [
+            ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, dp, dp, dp, ctypes.c_double, dp, dp, dp, dp, dp]
+        L.mo_pk1_from_pk2.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.mo_pk1_from_kirchhoff.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.mo_set_threads.argtypes = [ctypes.c_int]
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def _check(code):
+    if code != 0:
+        raise OracleError(code, lib().mo_last_error().decode())
+
+
+def set_threads(n):
+    lib().mo_set_threads(int(n))
+
+
+def hermitian_sizes(res):
+    res = list(res)
+    return res[:-1] + [res[-1] // 2 + 1]
+
+
+def fft_freqs(n, length=1.0):
+    out = np.zeros(n)
+    lib().mo_fft_freqs(n, float(length), _dp(out))
+    return out
+
+
+def fft_r2c(res, field):
+    res = np.ascontiguousarray(res, dtype=np.int32)
+    field = np.ascontiguousarray(field, dtype=np.float64)
+    nc = field.shape[-1]
+    kp = int(np.prod(hermitian_sizes(res)))
+    out = np.zeros((kp, nc), dtype=np.complex128)
+    _check(lib().mo_fft_r2c(len(res), _ip(res), nc, _dp(field),
+                            _dp(out.view(np.float64))))
+    return out
+
+
+def fft_c2r(res, spec):
+    res = np.ascontiguousarray(res, dtype=np.int32)
+    spec = np.ascontiguousarray(spec, dtype=np.complex128)
+    nc = spec.shape[-1]
+    out = np.zeros((int(np.prod(res)), nc))
+    _check(lib().mo_fft_c2r(len(res), _ip(res), nc,
+                            _dp(spec.view(np.float64)), _dp(out)))
+    return out
+
+
+def projection_operator(kind, res, lengths, allow_even=False):
+    d = len(res)
+    res_ = np.ascontiguousarray(res, dtype=np.int32)
+    L = np.ascontiguousarray(lengths, dtype=np.float64)
+    kp = int(np.prod(hermitian_sizes(res)))
+    nop = d if kind == PROJ_FINITE_FAST else d ** 4
+    out = np.zeros((kp, nop))
+    _check(lib().mo_projection_operator(kind, d, _ip(res_), _dp(L),
+                                        int(allow_even), _dp(out)))
+    return out
+
+
+def apply_projection(kind, res, lengths, field, allow_even=False):
+    d = len(res)
+    res_ = np.ascontiguousarray(res, dtype=np.int32)
+    L = np.ascontiguousarray(lengths, dtype=np.float64)
+    f = np.array(field, dtype=np.float64, order="C", copy=True)
+    _check(lib().mo_apply_projection(kind, d, _ip(res_), _dp(L),
+                                     int(allow_even), _dp(f)))
+    return f
+
+
+def _mat(d, fn, A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    out = np.zeros((d, d), order="F")
+    fn(d, _dp(A), _dp(out))
+    return out
+
+
+def logm(A):
+    return _mat(A.shape[0], lib().mo_logm, A)
+
+
+def logm_sylvester(A):
+    return _mat(A.shape[0], lib().mo_logm_sylvester, A)
+
+
+def expm(A):
+    return _mat(A.shape[0], lib().mo_expm, A)
+
+
+def expm_eig(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    out = np.zeros((3, 3), order="F")
+    lib().mo_expm_eig(_dp(A), _dp(out))
+    return out
+
+
+def eig_sym3(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    out = np.zeros(3)
+    lib().mo_eig_sym3(_dp(A), _dp(out))
+    return out
+
+
+def le1_eval(dim, form, young, poisson, F):
+    F = np.asfortranarray(F, dtype=np.float64)
+    P = np.zeros((dim, dim), order="F")
+    K = np.zeros(dim ** 4)
+    lib().mo_le1_eval(dim, form, young, poisson, _dp(F), _dp(P), _dp(K))
+    return P, K.reshape(dim * dim, dim * dim, order="F")
+
+
+def hep1_eval(dim, young, poisson, tau_y0, H, F, F_prev_old, be_old, eps_old,
+              eps_cur=0.0):
+    """Material-level HEP1 evaluation; returns (tau, C, F_prev_cur, be_cur,
+    eps_cur) with C as the dim^2 x dim^2 T4 matrix."""
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    F, Fp, be = f(F), f(F_prev_old), f(be_old)
+    Fpc = np.zeros((dim, dim), order="F")
+    bec = np.zeros((dim, dim), order="F")
+    epc = np.array([eps_cur], dtype=np.float64)
+    tau = np.zeros((dim, dim), order="F")
+    C = np.zeros(dim ** 4)
+    lib().mo_hep1_eval_kirchhoff(dim, young, poisson, tau_y0, H, _dp(F),
+                                 _dp(Fp), _dp(be), float(eps_old), _dp(Fpc),
+                                 _dp(bec), _dp(epc), _dp(tau), _dp(C))
+    return tau, C.reshape(dim * dim, dim * dim, order="F"), Fpc, bec, epc[0]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("success", ctypes.c_int), ("status", ctypes.c_int),
+                ("nb_it", ctypes.c_int), ("nb_fev", ctypes.c_long),
+                ("message", ctypes.c_int)]
+
+
+MESSAGES = ["Has not converged", "Residual  tolerance reached",
+            "Reached stress divergence tolerance"]
+
+
+class Cell:
+    """Oracle cell: same operations as the reference's CellBase."""
+
+    def __init__(self, res, lengths, form=FINITE_STRAIN, allow_even=False):
+        self.res = [int(r) for r in res]
+        self.dim = len(res)
+        self.nc = self.dim ** 2
+        self.N = int(np.prod(res))
+        self.form = form
+        r = np.ascontiguousarray(self.res, dtype=np.int32)
+        L = np.ascontiguousarray(lengths, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().mo_cell_create(self.dim, _ip(r), _dp(L), form,
+                                    int(allow_even), ctypes.byref(h)))
+        self._h = h
+        self.counter = ctypes.c_long(0)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().mo_cell_destroy(self._h)
+            self._h = None
+
+    def add_material(self, family, young=0.0, poisson=0.0, tau_y0=0.0, H=0.0):
+        m = lib().mo_cell_add_material(self._h, family, young, poisson,
+                                       tau_y0, H)
+        if m < 0:
+            _check(-m)
+        return m
+
+    def add_pixel(self, mat, pixel, params=None):
+        p = None
+        if params is not None:
+            arr = np.asfortranarray(params, dtype=np.float64).ravel(order="F")
+            self._keep = arr
+            p = _dp(arr)
+        _check(lib().mo_material_add_pixel(self._h, mat, int(pixel), p))
+
+    def add_pixels(self, mat, pixels, params=None):
+        for i, p in enumerate(pixels):
+            self.add_pixel(mat, p, None if params is None else params[i])
+
+    def initialise(self):
+        _check(lib().mo_cell_initialise(self._h))
+
+    def _view(self, fn, ncomp):
+        ptr = fn(self._h)
+        return np.ctypeslib.as_array(ptr, shape=(self.N, ncomp))
+
+    @property
+    def F(self):
+        return self._view(lib().mo_cell_strain, self.nc)
+
+    @property
+    def P(self):
+        return self._view(lib().mo_cell_stress, self.nc)
+
+    @property
+    def K(self):
+        return self._view(lib().mo_cell_tangent, self.nc * self.nc)
+
+    def evaluate_stress_tangent(self):
+        _check(lib().mo_cell_evaluate_stress_tangent(self._h))
+
+    def save_history(self):
+        _check(lib().mo_cell_save_history(self._h))
+
+    def directional_stiffness(self, dF):
+        dF = np.ascontiguousarray(dF, dtype=np.float64)
+        out = np.zeros_like(dF)
+        _check(lib().mo_cell_directional_stiffness(self._h, _dp(dF), _dp(out)))
+        return out
+
+    def tangent_product(self, dF):
+        dF = np.ascontiguousarray(dF, dtype=np.float64)
+        out = np.zeros_like(dF)
+        _check(lib().mo_cell_tangent_product(self._h, _dp(dF), _dp(out)))
+        return out
+
+    def apply_projection(self, v):
+        v = np.array(v, dtype=np.float64, order="C", copy=True)
+        _check(lib().mo_cell_apply_projection(self._h, _dp(v)))
+        return v
+
+    def get_internal(self, name, which=0):
+        out = np.zeros(self.N * self.nc)
+        nb = ctypes.c_int(0)
+        _check(lib().mo_cell_get_internal(self._h, name.encode(), which,
+                                          _dp(out), ctypes.byref(nb)))
+        return out[: self.N * nb.value].reshape(self.N, nb.value)
+
+    def cg_solve(self, rhs, tol, maxiter):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        _check(lib().mo_cg_solve(self._h, tol, maxiter, _dp(rhs), _dp(x),
+                                 ctypes.byref(self.counter)))
+        return x
+
+    def cg_fixed_iterations(self, rhs, iters):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        _check(lib().mo_cg_fixed_iterations(self._h, _dp(rhs), _dp(x), iters))
+        return x
+
+    def _drive(self, fn, load_steps, cg_tol, maxiter, newton_tol, equil_tol):
+        steps = [np.asarray(s, dtype=np.float64) for s in load_steps]
+        ls = np.concatenate([s.ravel(order="F") for s in steps])
+        n = len(steps)
+        res = (Result * n)()
+        grads = np.zeros((n, self.N, self.nc))
+        stresses = np.zeros((n, self.N, self.nc))
+        _check(fn(self._h, n, _dp(ls), cg_tol, maxiter, newton_tol, equil_tol,
+                  ctypes.byref(self.counter), ctypes.cast(res, ctypes.c_void_p),
+                  _dp(grads), _dp(stresses)))
+        out = []
+        for i in range(n):
+            out.append(dict(success=bool(res[i].success), status=res[i].status,
+                            nb_it=res[i].nb_it, nb_fev=res[i].nb_fev,
+                            message=MESSAGES[res[i].message],
+                            grad=grads[i], stress=stresses[i]))
+        return out
+
+    def newton_cg(self, load_steps, cg_tol, maxiter, newton_tol, equil_tol):
+        return self._drive(lib().mo_newton_cg, load_steps, cg_tol, maxiter,
+                           newton_tol, equil_tol)
+
+    def de_geus(self, load_steps, cg_tol, maxiter, newton_tol, equil_tol):
+        return self._drive(lib().mo_de_geus, load_steps, cg_tol, maxiter,
+                           newton_tol, equil_tol)
