@@ -1,0 +1,234 @@
+/*
+ * mugpu.h — C-ABI of the MI355X (gfx950) implementation of µSpectre's
+ * Newton-CG inner loop.  Plain pointers, sizes and opaque handles only.
+ *
+ * Every entry point names the reference interface it stands in for
+ * (paths relative to the reference repository root).  All functions return
+ * 0 on success or one of the MUGPU_E* codes; mugpu_last_error() gives the
+ * message.  The host layer maps the codes onto the reference's exception
+ * types (ProjectionError, SolverError, ConvergenceError, runtime_error).
+ *
+ * Array conventions are the reference's: real fields are AoS with nc
+ * components per pixel (tensor component A(i,j) at i + dim*j), pixels in
+ * row-major order with the last axis fastest; spectra are AoS complex over
+ * the Hermitian grid (N0, .., N_{d-1}/2+1)
+ * (src/common/field_typed.hh:119, src/common/ccoord_operations.hh:179-192,
+ *  src/fft/fftw_engine.cc:40-77).  Pointers named d_* are device pointers.
+ *
+ * precision: MUGPU_FP64 (plain path) or MUGPU_FP32 (operator stage in
+ * single precision).  Threading: calls on one handle are not thread-safe;
+ * all work runs on the library's stream, and the functions that return
+ * host scalars synchronise it.
+ */
+#ifndef MUGPU_H
+#define MUGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (src/solver/solver_common.hh:84-91,
+ *      src/fft/projection_base.hh:50-58) ---------------------------------- */
+#define MUGPU_OK 0
+#define MUGPU_ECONVERGENCE 1 /* ConvergenceError */
+#define MUGPU_ESOLVER 2      /* SolverError */
+#define MUGPU_EPROJECTION 3  /* ProjectionError */
+#define MUGPU_ERUNTIME 4     /* std::runtime_error */
+#define MUGPU_EMEMORY 5      /* device allocation failed */
+#define MUGPU_EHIP 6         /* HIP runtime error */
+#define MUGPU_EARG 7         /* invalid argument (size/shape mismatch) */
+
+#define MUGPU_FP64 0
+#define MUGPU_FP32 1
+
+/* src/common/common.hh:149-153 */
+#define MUGPU_FINITE_STRAIN 0
+#define MUGPU_SMALL_STRAIN 1
+
+/* projections: projection_small_strain.cc, projection_finite_strain_fast.cc,
+ * projection_finite_strain.cc */
+#define MUGPU_PROJ_SMALL_STRAIN 0
+#define MUGPU_PROJ_FINITE_STRAIN_FAST 1
+#define MUGPU_PROJ_FINITE_STRAIN 2
+
+/* material families: material_linear_elastic{1,2,3,4}.hh,
+ * material_hyper_elasto_plastic1.hh */
+#define MUGPU_LE1 1
+#define MUGPU_LE2 2
+#define MUGPU_LE3 3
+#define MUGPU_LE4 4
+#define MUGPU_HEP1 5
+
+typedef struct mugpu_fft_s *mugpu_fft;
+typedef struct mugpu_proj_s *mugpu_proj;
+typedef struct mugpu_cell_s *mugpu_cell;
+
+/* ---- runtime --------------------------------------------------------- */
+/* selects the device and creates the library stream; idempotent */
+int mugpu_init(int device);
+const char *mugpu_last_error(void);
+/* device name, CU count, HBM bytes, max LDS bytes per workgroup */
+int mugpu_device_info(char *name, int name_len, int *cu_count,
+                      size_t *hbm_bytes, int *lds_per_block);
+/* field memory: in the reference every field is a std::vector<Real> owned
+ * by a FieldCollection (src/common/field_typed.hh:443-449) */
+int mugpu_alloc(size_t bytes, void **d_ptr);
+int mugpu_free(void *d_ptr);
+int mugpu_upload(void *d_dst, const void *h_src, size_t bytes);
+int mugpu_download(void *h_dst, const void *d_src, size_t bytes);
+int mugpu_memset_zero(void *d_ptr, size_t bytes);
+int mugpu_synchronize(void);
+/* the library's stream (a hipStream_t), for callers that time kernels */
+void *mugpu_stream(void);
+
+/* ---- FFT engine ------------------------------------------------------
+ * FFTEngineBase ctor (src/fft/fft_engine_base.hh:75-76,
+ * fft_engine_base.cc:40-51) + FFTWEngine::initialise
+ * (src/fft/fftw_engine.cc:52-117): batched r2c/c2r of nb_components
+ * interleaved components (howmany=nc, stride nc). */
+int mugpu_fft_create(int dim, const int *resolutions, int nb_components,
+                     int precision, mugpu_fft *out);
+int mugpu_fft_destroy(mugpu_fft plan);
+/* FFTWEngine::fft (src/fft/fftw_engine.cc:130-143): unnormalised r2c,
+ * d_real (N*nc reals) -> d_spectrum (Kpts*nc complex); input preserved */
+int mugpu_fft_r2c(mugpu_fft plan, const void *d_real, void *d_spectrum);
+/* FFTWEngine::ifft (src/fft/fftw_engine.cc:146-158): unnormalised c2r;
+ * the spectrum is overwritten, as with the reference's FFTW c2r plan */
+int mugpu_fft_c2r(mugpu_fft plan, void *d_spectrum, void *d_real);
+/* FFTEngineBase::normalisation (src/fft/fft_engine_base.hh:145): 1/N */
+double mugpu_fft_normalisation(mugpu_fft plan);
+/* number of Hermitian k-points, N0*..*(N_{d-1}/2+1)
+ * (src/common/ccoord_operations.hh:101-104) */
+int64_t mugpu_fft_nb_fourier_points(mugpu_fft plan);
+
+/* ---- projection ------------------------------------------------------
+ * ProjectionSmallStrain / ProjectionFiniteStrainFast / ProjectionFiniteStrain
+ * ctor + initialise (src/fft/projection_small_strain.cc:42-86,
+ * projection_finite_strain_fast.cc:43-73, projection_finite_strain.cc:45-95).
+ * The operator is generated from the wave-vector indices inside the kernel;
+ * nothing is tabulated.  allow_even = 0 keeps the reference's odd-only
+ * check (MUGPU_EPROJECTION); 1 zeroes the operator on every Nyquist index
+ * of even axes.  The engine is owned by the projection afterwards
+ * (src/fft/projection_base.cc:41-53) and destroyed with it. */
+int mugpu_projection_create(mugpu_fft engine, int kind, const double *lengths,
+                            int allow_even, mugpu_proj *out);
+int mugpu_projection_destroy(mugpu_proj proj);
+/* ProjectionBase::apply_projection (src/fft/projection_base.hh:120,
+ * projection_default.cc:53-63, projection_finite_strain_fast.cc:77-87):
+ * in place on d_field (N*nc reals, plan precision) */
+int mugpu_projection_apply(mugpu_proj proj, void *d_field);
+/* ProjectionBase::get_operator (src/fft/projection_base.hh:144): host copy
+ * of the reference's stored operator, Kpts x (nc*nc) for the full-G
+ * projections and Kpts x dim for the fast finite-strain one */
+int mugpu_projection_get_operator(mugpu_proj proj, double *h_out);
+
+/* ---- cell -------------------------------------------------------------
+ * make_cell / CellBase (src/cell/cell_factory.hh:59-105,
+ * src/cell/cell_base.cc:48-62): owns F ("Gradient"), P ("Piola-Kirchhoff-1"),
+ * the tangent, the projection of the formulation (finite strain: fast
+ * projection, small strain: small-strain projection), and materials.
+ * precision selects the operator stage (tangent product, FFT, projection). */
+int mugpu_cell_create(int dim, const int *resolutions, const double *lengths,
+                      int formulation, int precision, int allow_even,
+                      mugpu_cell *out);
+int mugpu_cell_destroy(mugpu_cell cell);
+int64_t mugpu_cell_nb_pixels(mugpu_cell cell);
+int mugpu_cell_nb_components(mugpu_cell cell);
+/* Material::make (src/materials/material_muSpectre_base.hh:228-237):
+ * returns the material index in *mat.  young/poisson for LE1, LE2, HEP1;
+ * tau_y0 and H for HEP1 (material_hyper_elasto_plastic1.cc:40-65) */
+int mugpu_cell_add_material(mugpu_cell cell, int family, double young,
+                            double poisson, double tau_y0, double H,
+                            int *mat);
+/* MaterialBase::add_pixel and the LE2-LE4 overloads
+ * (src/materials/material_base.cc:56-58, material_linear_elastic2.cc:57-64,
+ * material_linear_elastic3.cc:57-68, material_linear_elastic4.cc:62-73):
+ * pixels are linear indices; params per pixel: LE2 the eigenstrain
+ * (dim*dim), LE3/LE4 (Young, Poisson); NULL otherwise */
+int mugpu_material_add_pixels(mugpu_cell cell, int mat, int64_t count,
+                              const int64_t *pixels, const double *params);
+/* CellBase::initialise (src/cell/cell_base.cc:445-454): coverage check
+ * (:486-526), material initialise, projection initialise */
+int mugpu_cell_initialise(mugpu_cell cell);
+int mugpu_cell_is_initialised(mugpu_cell cell);
+/* device pointers of F and P (FP64, N*nc): the fields behind
+ * Cell::get_strain_vector / get_stress_vector (src/cell/cell_base.hh:148) */
+int mugpu_cell_strain(mugpu_cell cell, void **d_F);
+int mugpu_cell_stress(mugpu_cell cell, void **d_P);
+/* CellBase::set_uniform_strain (src/cell/cell_base.cc:102-106); h_strain is
+ * dim*dim, column-major */
+int mugpu_cell_set_uniform_strain(mugpu_cell cell, const double *h_strain);
+/* F += h_increment on every pixel (src/solver/solvers.cc:142-145) */
+int mugpu_cell_add_uniform_strain(mugpu_cell cell, const double *h_increment);
+/* CellBase::evaluate_stress_tangent (src/cell/cell_base.cc:129-146):
+ * every material on its own pixels; P and the tangent are updated */
+int mugpu_cell_evaluate_stress_tangent(mugpu_cell cell);
+/* CellBase::evaluate_stress (src/cell/cell_base.cc:112-124) */
+int mugpu_cell_evaluate_stress(mugpu_cell cell);
+/* copy of the tangent as the reference stores it, N x (nc*nc) FP64
+ * (src/cell/cell_base.cc:323-334) */
+int mugpu_cell_get_tangent(mugpu_cell cell, double *h_K);
+/* CellBase::evaluate_projected_directional_stiffness
+ * (src/cell/cell_base.cc:150-192): d_out = G (K : d_in); FP64 vectors */
+int mugpu_cell_directional_stiffness(mugpu_cell cell, const void *d_in,
+                                     void *d_out);
+/* K : d_in only, no projection (src/cell/cell_base.cc:182-188) */
+int mugpu_cell_tangent_product(mugpu_cell cell, const void *d_in,
+                               void *d_out);
+/* CellBase::apply_projection (src/cell/cell_base.cc:201-208), FP64 vector */
+int mugpu_cell_apply_projection(mugpu_cell cell, void *d_vec);
+/* CellBase::save_history_variables (src/cell/cell_base.cc:458-462) ->
+ * StateField::cycle (src/common/statefield.hh:322-326) */
+int mugpu_cell_save_history(mugpu_cell cell);
+/* CellBase::get_globalised_internal_real_array (src/cell/cell_base.cc:
+ * 371-441): "eps_p", "F_prev", "be" (HEP1, current values) or
+ * "eigenstrain" (LE2); pixels without the field get zeros */
+int mugpu_cell_get_internal(mugpu_cell cell, const char *name, double *h_out,
+                            int *nb_components);
+
+/* ---- vector kernels for the Newton driver (src/solver/solvers.cc:
+ * 143-145,173-175,184-187; Communicator::sum is the identity on one GPU,
+ * src/common/communicator.hh:122-124) ------------------------------------ */
+/* sum of squares / dot over n doubles, deterministic order */
+int mugpu_dot(const void *d_a, const void *d_b, int64_t n, double *h_out);
+/* d_y = a*d_x + b*d_y over n doubles */
+int mugpu_axpby(double a, const void *d_x, double b, void *d_y, int64_t n);
+/* d_y = a*d_x */
+int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
+
+/* ---- SolverCG (src/solver/solver_cg.cc:54-110) -----------------------
+ * Device-resident CG.  The solver owns x, r, p, Ap (solver_cg.cc:45-51);
+ * the host loop keeps the reference's control flow and reads back one
+ * scalar per iteration.
+ * mugpu_cg_start: x = 0; r = A x - b (one full operator application);
+ *   p = -r; returns rdr = r.r and rhs_norm2 = b.b.
+ * mugpu_cg_step: [if update_p: p = -r + beta p with beta from the previous
+ *   step] Ap = A p; alpha = rdr/(p.Ap); x += alpha p; r += alpha Ap;
+ *   new_rdr = r.r; beta = new_rdr/rdr.  Returns new_rdr.
+ * In FP32 cell precision the operator runs in FP32 on the FP64 vectors. */
+int mugpu_cg_start(mugpu_cell cell, const void *d_rhs, void *d_x, void *d_r,
+                   void *d_p, void *d_Ap, double *rdr, double *rhs_norm2);
+int mugpu_cg_step(mugpu_cell cell, int update_p, void *d_x, void *d_r,
+                  void *d_p, void *d_Ap, double *new_rdr);
+
+/* ---- mixed-precision CG with FP64 refinement (no reference counterpart;
+ * it replaces SolverCG::solve for cells created with MUGPU_FP32) --------
+ * Inner CG on FP32 vectors with the FP32 operator and FP64 dot products
+ * and alpha/beta; the outer loop recomputes the true residual b - A x with
+ * the FP64 operator and corrects x in FP64.  Stops when
+ * |b - A x|^2 <= tol^2 |b|^2 or after maxiter inner iterations in total.
+ * *iterations returns the inner CG iterations, *refinements the outer
+ * FP64 residual evaluations.  Returns MUGPU_ECONVERGENCE when not
+ * converged. */
+int mugpu_cg_solve_mixed(mugpu_cell cell, const void *d_rhs, void *d_x,
+                         double tol, int maxiter, double inner_tol,
+                         int64_t *iterations, int *refinements,
+                         double *rel_residual);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,861 @@
+// C-ABI of libmugpu.so (declared in include/mugpu.h).  Handles wrap device
+// buffers; every function reports errors through return codes and
+// mugpu_last_error().  Work runs on one library stream.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "mugpu_internal.hpp"
+
+namespace mugpu {
+
+namespace {
+char g_err[2048] = "";
+hipStream_t g_stream = nullptr;
+int g_device = -1;
+int g_maxlds = 65536;
+double* g_host_scal = nullptr;  // pinned
+double* g_dev_scratch = nullptr;  // partials + scalars for mugpu_dot
+}  // namespace
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+  return MUGPU_EHIP;
+}
+
+hipStream_t stream() { return g_stream; }
+int max_lds_per_block() { return g_maxlds; }
+
+int run_tangent_product(int dim, int prec, int kind, const void* tan, const void* in, void* out,
+                        int64_t npix);
+
+}  // namespace mugpu
+
+using namespace mugpu;
+
+struct mugpu_fft_s {
+  FftPlan plan;
+};
+
+struct mugpu_proj_s {
+  mugpu_fft engine = nullptr;
+  ProjParams pj;
+  void* spectrum = nullptr;
+};
+
+struct mugpu_cell_s {
+  int dim = 0, form = 0, prec = 0, allow_even = 0, nc = 0, initialised = 0;
+  int res[3] = {1, 1, 1};
+  double L[3] = {1, 1, 1};
+  int64_t N = 0;
+  FftPlan plan;
+  ProjParams pj;
+  double* F = nullptr;
+  double* P = nullptr;
+  int tan_kind = TAN_FULL;
+  double* tan64 = nullptr;
+  float* tan32 = nullptr;
+  void* spectrum = nullptr;
+  double* tmp = nullptr;
+  double* scal = nullptr;
+  double* partials = nullptr;
+  std::vector<MaterialDev> mats;
+  float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr;
+  double* r64 = nullptr;
+};
+
+namespace {
+
+int ensure_init() {
+  if (g_device < 0) return mugpu_init(0);
+  return MUGPU_OK;
+}
+
+int sync_scalars(const double* d_scal) {
+  MUGPU_HIP(hipMemcpyAsync(g_host_scal, d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost,
+                           g_stream));
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  return MUGPU_OK;
+}
+
+int check_odd(int dim, const int* res, int allow_even) {
+  if (allow_even) return MUGPU_OK;
+  for (int d = 0; d < dim; ++d)
+    if (res[d] % 2 == 0) {
+      set_error("Only an odd number of gridpoints in each direction is supported");
+      return MUGPU_EPROJECTION;
+    }
+  return MUGPU_OK;
+}
+
+double lame_lambda(double E, double nu) { return E * nu / ((1 + nu) * (1 - 2 * nu)); }
+double lame_mu(double E, double nu) { return E / (2 * (1 + nu)); }
+double bulk_modulus(double E, double nu) { return E / (3 * (1 - 2 * nu)); }
+
+template <typename P>
+void free_dev(P*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+// host evaluation of the stored operator for get_operator (debug view)
+void host_operator(int kind, int dim, const int* res, const double* L, int allow_even,
+                   double* out) {
+  int h[3] = {1, 1, 1};
+  for (int d = 0; d < dim; ++d) h[d] = res[d];
+  h[dim - 1] = res[dim - 1] / 2 + 1;
+  int64_t kp = 1;
+  for (int d = 0; d < dim; ++d) kp *= h[d];
+  const int nc = dim * dim;
+  const int nop = kind == MUGPU_PROJ_FINITE_STRAIN_FAST ? dim : nc * nc;
+  for (int64_t k = 0; k < kp; ++k) {
+    int c[3];
+    int64_t r = k;
+    for (int d = dim - 1; d >= 0; --d) {
+      c[d] = (int)(r % h[d]);
+      r /= h[d];
+    }
+    double xi[3], n2 = 0;
+    bool zero = k == 0, nyq = false;
+    for (int d = 0; d < dim; ++d) {
+      const int npos = (res[d] - 1) / 2 + 1;
+      xi[d] = (double)(c[d] < npos ? c[d] : c[d] - res[d]) / L[d];
+      n2 += xi[d] * xi[d];
+      if (allow_even && res[d] % 2 == 0 && 2 * c[d] == res[d]) nyq = true;
+    }
+    double* o = out + k * nop;
+    if (zero || nyq) {
+      for (int q = 0; q < nop; ++q) o[q] = 0.0;
+      continue;
+    }
+    const double nn = std::sqrt(n2);
+    for (int d = 0; d < dim; ++d) xi[d] /= nn;
+    if (kind == MUGPU_PROJ_FINITE_STRAIN_FAST) {
+      for (int d = 0; d < dim; ++d) o[d] = xi[d];
+      continue;
+    }
+    for (int i = 0; i < dim; ++i)
+      for (int j = 0; j < dim; ++j)
+        for (int l = 0; l < dim; ++l)
+          for (int m = 0; m < dim; ++m) {
+            double g;
+            if (kind == MUGPU_PROJ_SMALL_STRAIN)
+              g = 0.5 * (xi[i] * (j == l) * xi[m] + xi[i] * (j == m) * xi[l] +
+                         xi[j] * (i == l) * xi[m] + xi[j] * (i == m) * xi[l]) -
+                  xi[i] * xi[j] * xi[l] * xi[m];
+            else
+              g = (i == l) * xi[j] * xi[m];
+            o[(i + dim * j) + nc * (l + dim * m)] = g;
+          }
+  }
+}
+
+int cell_operator64(mugpu_cell c, const void* in, void* out) {
+  OpSpec op;
+  op.in_mode = IN_TAN;
+  op.in = in;
+  op.tan_kind = c->tan_kind;
+  op.tangent = c->tan64;
+  op.out_mode = OUT_PLAIN;
+  op.out = out;
+  return run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
+}
+
+}  // namespace
+
+// ============================== runtime ======================================
+extern "C" int mugpu_init(int device) {
+  if (g_device >= 0) return MUGPU_OK;
+  int count = 0;
+  MUGPU_HIP(hipGetDeviceCount(&count));
+  if (count <= 0) {
+    set_error("no HIP device");
+    return MUGPU_EHIP;
+  }
+  if (device < 0 || device >= count) {
+    set_error("device %d out of range (%d devices)", device, count);
+    return MUGPU_EARG;
+  }
+  MUGPU_HIP(hipSetDevice(device));
+  MUGPU_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  MUGPU_HIP(hipDeviceGetAttribute(&g_maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  MUGPU_HIP(hipHostMalloc((void**)&g_host_scal, sizeof(double) * SC_COUNT, hipHostMallocDefault));
+  MUGPU_HIP(hipMalloc((void**)&g_dev_scratch, sizeof(double) * (kRedBlocks + SC_COUNT)));
+  g_device = device;
+  return MUGPU_OK;
+}
+
+extern "C" const char* mugpu_last_error(void) { return g_err; }
+
+extern "C" int mugpu_device_info(char* name, int name_len, int* cu_count, size_t* hbm_bytes,
+                                 int* lds_per_block) {
+  MUGPU_TRY(ensure_init());
+  hipDeviceProp_t prop;
+  MUGPU_HIP(hipGetDeviceProperties(&prop, g_device));
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+  if (lds_per_block) *lds_per_block = g_maxlds;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_alloc(size_t bytes, void** d_ptr) {
+  MUGPU_TRY(ensure_init());
+  *d_ptr = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(d_ptr, bytes);
+  if (e != hipSuccess) {
+    set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    return MUGPU_EMEMORY;
+  }
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_free(void* d_ptr) {
+  if (d_ptr) MUGPU_HIP(hipFree(d_ptr));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_upload(void* d_dst, const void* h_src, size_t bytes) {
+  MUGPU_TRY(ensure_init());
+  MUGPU_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, g_stream));
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_download(void* h_dst, const void* d_src, size_t bytes) {
+  MUGPU_TRY(ensure_init());
+  MUGPU_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, g_stream));
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_memset_zero(void* d_ptr, size_t bytes) {
+  MUGPU_TRY(ensure_init());
+  MUGPU_HIP(hipMemsetAsync(d_ptr, 0, bytes, g_stream));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_synchronize(void) {
+  MUGPU_TRY(ensure_init());
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  return MUGPU_OK;
+}
+
+extern "C" void* mugpu_stream(void) { return (void*)g_stream; }
+
+// ============================== FFT engine ====================================
+extern "C" int mugpu_fft_create(int dim, const int* res, int nb_components, int precision,
+                                mugpu_fft* out) {
+  MUGPU_TRY(ensure_init());
+  *out = nullptr;
+  mugpu_fft f = new (std::nothrow) mugpu_fft_s;
+  if (!f) return MUGPU_EMEMORY;
+  int rc = plan_init(f->plan, dim, res, nb_components, precision);
+  if (rc) {
+    plan_free(f->plan);
+    delete f;
+    return rc;
+  }
+  *out = f;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_fft_destroy(mugpu_fft plan) {
+  if (!plan) return MUGPU_OK;
+  plan_free(plan->plan);
+  delete plan;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_fft_r2c(mugpu_fft plan, const void* d_real, void* d_spectrum) {
+  if (!plan) return MUGPU_EARG;
+  return run_r2c(plan->plan, plan->plan.prec, d_real, d_spectrum);
+}
+
+extern "C" int mugpu_fft_c2r(mugpu_fft plan, void* d_spectrum, void* d_real) {
+  if (!plan) return MUGPU_EARG;
+  return run_c2r(plan->plan, plan->plan.prec, d_spectrum, d_real);
+}
+
+extern "C" double mugpu_fft_normalisation(mugpu_fft plan) {
+  return plan ? 1.0 / (double)plan->plan.N : 0.0;
+}
+
+extern "C" int64_t mugpu_fft_nb_fourier_points(mugpu_fft plan) {
+  return plan ? plan->plan.kpts : 0;
+}
+
+// ============================== projection ====================================
+extern "C" int mugpu_projection_create(mugpu_fft engine, int kind, const double* lengths,
+                                       int allow_even, mugpu_proj* out) {
+  *out = nullptr;
+  if (!engine) return MUGPU_EARG;
+  if (kind < MUGPU_PROJ_SMALL_STRAIN || kind > MUGPU_PROJ_FINITE_STRAIN) {
+    set_error("unknown projection kind %d", kind);
+    return MUGPU_EARG;
+  }
+  MUGPU_TRY(check_odd(engine->plan.dim, engine->plan.res, allow_even));
+  mugpu_proj p = new (std::nothrow) mugpu_proj_s;
+  if (!p) return MUGPU_EMEMORY;
+  p->engine = engine;
+  p->pj.kind = kind;
+  p->pj.allow_even = allow_even;
+  for (int d = 0; d < engine->plan.dim; ++d) p->pj.L[d] = lengths[d];
+  p->pj.norm = 1.0 / (double)engine->plan.N;
+  const size_t bytes = (size_t)engine->plan.kpts * engine->plan.nc * 2 *
+                       (engine->plan.prec == MUGPU_FP64 ? sizeof(double) : sizeof(float));
+  if (hipMalloc(&p->spectrum, bytes) != hipSuccess) {
+    delete p;
+    set_error("device allocation of the projection work space failed");
+    return MUGPU_EMEMORY;
+  }
+  *out = p;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_projection_destroy(mugpu_proj proj) {
+  if (!proj) return MUGPU_OK;
+  free_dev(proj->spectrum);
+  mugpu_fft_destroy(proj->engine);
+  delete proj;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_projection_apply(mugpu_proj proj, void* d_field) {
+  if (!proj) return MUGPU_EARG;
+  OpSpec op;
+  op.in_mode = IN_PLAIN;
+  op.in = d_field;
+  op.out_mode = OUT_PLAIN;
+  op.out = d_field;
+  return run_operator(proj->engine->plan, proj->engine->plan.prec, op, proj->pj, proj->spectrum);
+}
+
+extern "C" int mugpu_projection_get_operator(mugpu_proj proj, double* h_out) {
+  if (!proj) return MUGPU_EARG;
+  const FftPlan& p = proj->engine->plan;
+  host_operator(proj->pj.kind, p.dim, p.res, proj->pj.L, proj->pj.allow_even, h_out);
+  return MUGPU_OK;
+}
+
+// ============================== cell ==========================================
+extern "C" int mugpu_cell_create(int dim, const int* res, const double* lengths, int formulation,
+                                 int precision, int allow_even, mugpu_cell* out) {
+  MUGPU_TRY(ensure_init());
+  *out = nullptr;
+  if (formulation != MUGPU_FINITE_STRAIN && formulation != MUGPU_SMALL_STRAIN) {
+    set_error("unknow formulation");
+    return MUGPU_ERUNTIME;
+  }
+  MUGPU_TRY(check_odd(dim, res, allow_even));
+  mugpu_cell c = new (std::nothrow) mugpu_cell_s;
+  if (!c) return MUGPU_EMEMORY;
+  c->dim = dim;
+  c->form = formulation;
+  c->prec = precision;
+  c->allow_even = allow_even;
+  c->nc = dim * dim;
+  int rc = plan_init(c->plan, dim, res, c->nc, MUGPU_FP64);
+  if (rc) {
+    plan_free(c->plan);
+    delete c;
+    return rc;
+  }
+  c->N = c->plan.N;
+  for (int d = 0; d < dim; ++d) {
+    c->res[d] = res[d];
+    c->L[d] = lengths[d];
+    c->pj.L[d] = lengths[d];
+  }
+  c->pj.kind = formulation == MUGPU_SMALL_STRAIN ? MUGPU_PROJ_SMALL_STRAIN
+                                                 : MUGPU_PROJ_FINITE_STRAIN_FAST;
+  c->pj.allow_even = allow_even;
+  c->pj.norm = 1.0 / (double)c->N;
+  const size_t fb = sizeof(double) * (size_t)c->N * c->nc;
+  const size_t sb = sizeof(double) * 2 * (size_t)c->plan.kpts * c->nc;
+  bool ok = hipMalloc((void**)&c->F, fb) == hipSuccess && hipMalloc((void**)&c->P, fb) == hipSuccess &&
+            hipMalloc((void**)&c->tmp, fb) == hipSuccess && hipMalloc(&c->spectrum, sb) == hipSuccess &&
+            hipMalloc((void**)&c->scal, sizeof(double) * SC_COUNT) == hipSuccess &&
+            hipMalloc((void**)&c->partials, sizeof(double) * kRedBlocks) == hipSuccess;
+  if (!ok) {
+    mugpu_cell_destroy(c);
+    set_error("device allocation of the cell fields failed");
+    return MUGPU_EMEMORY;
+  }
+  (void)hipMemsetAsync(c->F, 0, fb, g_stream);
+  (void)hipMemsetAsync(c->P, 0, fb, g_stream);
+  (void)hipMemsetAsync(c->scal, 0, sizeof(double) * SC_COUNT, g_stream);
+  *out = c;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_destroy(mugpu_cell c) {
+  if (!c) return MUGPU_OK;
+  (void)hipStreamSynchronize(g_stream);
+  for (auto& m : c->mats) {
+    free_dev(m.d_pix);
+    free_dev(m.d_params);
+    for (int b = 0; b < 2; ++b) {
+      free_dev(m.Fp[b]);
+      free_dev(m.be[b]);
+      free_dev(m.ep[b]);
+    }
+  }
+  free_dev(c->F);
+  free_dev(c->P);
+  free_dev(c->tan64);
+  free_dev(c->tan32);
+  free_dev(c->spectrum);
+  free_dev(c->tmp);
+  free_dev(c->scal);
+  free_dev(c->partials);
+  free_dev(c->x32);
+  free_dev(c->r32);
+  free_dev(c->p32);
+  free_dev(c->Ap32);
+  free_dev(c->r64);
+  plan_free(c->plan);
+  delete c;
+  return MUGPU_OK;
+}
+
+extern "C" int64_t mugpu_cell_nb_pixels(mugpu_cell c) { return c ? c->N : 0; }
+extern "C" int mugpu_cell_nb_components(mugpu_cell c) { return c ? c->nc : 0; }
+
+extern "C" int mugpu_cell_add_material(mugpu_cell c, int family, double young, double poisson,
+                                       double tau_y0, double H, int* mat) {
+  if (!c) return MUGPU_EARG;
+  if (c->initialised) {
+    set_error("materials must be added before the cell is initialised");
+    return MUGPU_ERUNTIME;
+  }
+  if (family < MUGPU_LE1 || family > MUGPU_HEP1) {
+    set_error("unknown material family %d", family);
+    return MUGPU_EARG;
+  }
+  if (family == MUGPU_HEP1 && c->form != MUGPU_FINITE_STRAIN) {
+    set_error("MaterialHyperElastoPlastic1 needs the finite-strain formulation");
+    return MUGPU_ERUNTIME;
+  }
+  MaterialDev m;
+  m.family = family;
+  m.young = young;
+  m.poisson = poisson;
+  m.tau_y0 = tau_y0;
+  m.H = H;
+  m.lambda = lame_lambda(young, poisson);
+  m.mu = lame_mu(young, poisson);
+  m.bulk = bulk_modulus(young, poisson);
+  m.ppn = family == MUGPU_LE2 ? c->nc : (family == MUGPU_LE3 || family == MUGPU_LE4) ? 2 : 0;
+  c->mats.push_back(std::move(m));
+  *mat = (int)c->mats.size() - 1;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_material_add_pixels(mugpu_cell c, int mat, int64_t count,
+                                         const int64_t* pixels, const double* params) {
+  if (!c || mat < 0 || mat >= (int)c->mats.size()) return MUGPU_EARG;
+  if (c->initialised) {
+    set_error("pixels must be added before the cell is initialised");
+    return MUGPU_ERUNTIME;
+  }
+  MaterialDev& m = c->mats[mat];
+  if (m.ppn && !params) {
+    set_error(m.family == MUGPU_LE2 ? "this material needs pixels with and eigenstrain"
+                                    : "this material needs pixels with Youngs modulus and Poisson ratio.");
+    return MUGPU_ERUNTIME;
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    if (pixels[i] < 0 || pixels[i] >= c->N) {
+      set_error("pixel index %lld outside the grid", (long long)pixels[i]);
+      return MUGPU_EARG;
+    }
+  }
+  for (int64_t i = 0; i < count; ++i) {
+    m.h_pix.push_back(pixels[i]);
+    if (m.family == MUGPU_LE2) {
+      for (int k = 0; k < c->nc; ++k) m.h_params.push_back(params[i * c->nc + k]);
+    } else if (m.family == MUGPU_LE3 || m.family == MUGPU_LE4) {
+      const double E = params[2 * i], nu = params[2 * i + 1];
+      m.h_params.push_back(lame_lambda(E, nu));
+      m.h_params.push_back(lame_mu(E, nu));
+    }
+  }
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_initialise(mugpu_cell c) {
+  if (!c) return MUGPU_EARG;
+  if (c->initialised) return MUGPU_OK;
+  // CellBase::check_material_coverage (src/cell/cell_base.cc:486-526)
+  std::vector<int> assign((size_t)c->N, -1);
+  for (size_t mi = 0; mi < c->mats.size(); ++mi)
+    for (int64_t p : c->mats[mi].h_pix) {
+      if (assign[(size_t)p] >= 0) {
+        set_error("Pixel %lld is already assigned to material %d and cannot be reassigned to material %d",
+                  (long long)p, assign[(size_t)p], (int)mi);
+        return MUGPU_ERUNTIME;
+      }
+      assign[(size_t)p] = (int)mi;
+    }
+  for (int64_t i = 0; i < c->N; ++i)
+    if (assign[(size_t)i] < 0) {
+      set_error("The following pixels have were not assigned a material: %lld, ... and that cannot be handled",
+                (long long)i);
+      return MUGPU_ERUNTIME;
+    }
+  // tangent representation
+  bool all_hep = !c->mats.empty();
+  for (auto& m : c->mats) all_hep = all_hep && m.family == MUGPU_HEP1;
+  c->tan_kind = c->form == MUGPU_SMALL_STRAIN ? TAN_ISO : (all_hep ? TAN_HEP : TAN_FULL);
+  const size_t tn = (size_t)c->N * tangent_stride(c->tan_kind, c->dim);
+  if (hipMalloc((void**)&c->tan64, tn * sizeof(double)) != hipSuccess) {
+    set_error("device allocation of the tangent failed");
+    return MUGPU_EMEMORY;
+  }
+  MUGPU_HIP(hipMemsetAsync(c->tan64, 0, tn * sizeof(double), g_stream));
+  if (c->prec == MUGPU_FP32) {
+    if (hipMalloc((void**)&c->tan32, tn * sizeof(float)) != hipSuccess) {
+      set_error("device allocation of the FP32 tangent failed");
+      return MUGPU_EMEMORY;
+    }
+  }
+  const int nc = c->nc;
+  for (auto& m : c->mats) {
+    const int64_t n = (int64_t)m.h_pix.size();
+    std::vector<int> pix32(m.h_pix.begin(), m.h_pix.end());
+    MUGPU_HIP(hipMalloc((void**)&m.d_pix, sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    if (n) MUGPU_HIP(hipMemcpy(m.d_pix, pix32.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (m.ppn) {
+      MUGPU_HIP(hipMalloc((void**)&m.d_params, sizeof(double) * (size_t)(n > 0 ? n * m.ppn : 1)));
+      if (n) MUGPU_HIP(hipMemcpy(m.d_params, m.h_params.data(), sizeof(double) * n * m.ppn,
+                                 hipMemcpyHostToDevice));
+    }
+    if (m.family == MUGPU_HEP1) {
+      // MaterialHyperElastoPlastic1::initialise (.cc:76-84): current = I,
+      // then cycle; both buffers start at zero
+      const size_t sz = (size_t)(n > 0 ? n : 1);
+      for (int b = 0; b < 2; ++b) {
+        MUGPU_HIP(hipMalloc((void**)&m.Fp[b], sizeof(double) * sz * nc));
+        MUGPU_HIP(hipMalloc((void**)&m.be[b], sizeof(double) * sz * nc));
+        MUGPU_HIP(hipMalloc((void**)&m.ep[b], sizeof(double) * sz));
+        MUGPU_HIP(hipMemsetAsync(m.Fp[b], 0, sizeof(double) * sz * nc, g_stream));
+        MUGPU_HIP(hipMemsetAsync(m.be[b], 0, sizeof(double) * sz * nc, g_stream));
+        MUGPU_HIP(hipMemsetAsync(m.ep[b], 0, sizeof(double) * sz, g_stream));
+      }
+      std::vector<double> eye(nc, 0.0);
+      for (int i = 0; i < c->dim; ++i) eye[i * (c->dim + 1)] = 1.0;
+      MUGPU_TRY(fill_uniform(m.Fp[0], eye.data(), nc, n, 0));
+      MUGPU_TRY(fill_uniform(m.be[0], eye.data(), nc, n, 0));
+      m.cur = 1;
+    }
+  }
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  c->initialised = 1;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_is_initialised(mugpu_cell c) { return c ? c->initialised : 0; }
+
+extern "C" int mugpu_cell_strain(mugpu_cell c, void** d_F) {
+  if (!c) return MUGPU_EARG;
+  *d_F = c->F;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_stress(mugpu_cell c, void** d_P) {
+  if (!c) return MUGPU_EARG;
+  *d_P = c->P;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_set_uniform_strain(mugpu_cell c, const double* h_strain) {
+  if (!c) return MUGPU_EARG;
+  return fill_uniform(c->F, h_strain, c->nc, c->N, 0);
+}
+
+extern "C" int mugpu_cell_add_uniform_strain(mugpu_cell c, const double* h_increment) {
+  if (!c) return MUGPU_EARG;
+  return fill_uniform(c->F, h_increment, c->nc, c->N, 1);
+}
+
+static int evaluate(mugpu_cell c, int need_tangent) {
+  if (!c) return MUGPU_EARG;
+  MUGPU_TRY(mugpu_cell_initialise(c));
+  for (auto& m : c->mats)
+    MUGPU_TRY(eval_material(m, c->dim, c->form, c->F, c->P, c->tan_kind, c->tan64, need_tangent));
+  if (need_tangent && c->prec == MUGPU_FP32) {
+    const int64_t tn = c->N * tangent_stride(c->tan_kind, c->dim);
+    MUGPU_TRY(convert(MUGPU_FP64, c->tan64, MUGPU_FP32, c->tan32, tn, 1.0));
+  }
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_evaluate_stress_tangent(mugpu_cell c) { return evaluate(c, 1); }
+extern "C" int mugpu_cell_evaluate_stress(mugpu_cell c) { return evaluate(c, 0); }
+
+extern "C" int mugpu_cell_get_tangent(mugpu_cell c, double* h_K) {
+  if (!c || !c->tan64) {
+    set_error("K does not exist");
+    return MUGPU_ERUNTIME;
+  }
+  const int d = c->dim, nc = c->nc, st = tangent_stride(c->tan_kind, d);
+  std::vector<double> t((size_t)c->N * st);
+  MUGPU_TRY(mugpu_download(t.data(), c->tan64, t.size() * sizeof(double)));
+  for (int64_t v = 0; v < c->N; ++v) {
+    double* K = h_K + v * nc * nc;
+    const double* s = t.data() + v * st;
+    if (c->tan_kind == TAN_FULL) {
+      std::memcpy(K, s, sizeof(double) * nc * nc);
+    } else if (c->tan_kind == TAN_ISO) {
+      for (int l = 0; l < d; ++l)
+        for (int k = 0; k < d; ++k)
+          for (int j = 0; j < d; ++j)
+            for (int i = 0; i < d; ++i) {
+              const double itrac = (i == j && k == l) ? 1.0 : 0.0;
+              const double isym = 0.5 * (((i == k && j == l) ? 1.0 : 0.0) + ((i == l && j == k) ? 1.0 : 0.0));
+              K[(i + d * j) + nc * (k + d * l)] = s[0] * itrac + 2 * s[1] * isym;
+            }
+    } else {
+      for (int n = 0; n < d; ++n)
+        for (int j = 0; j < d; ++j)
+          for (int m = 0; m < d; ++m)
+            for (int i = 0; i < d; ++i) K[(i + d * m) + nc * (j + d * n)] = s[i + d * m + d * d * n];
+    }
+  }
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_directional_stiffness(mugpu_cell c, const void* d_in, void* d_out) {
+  if (!c) return MUGPU_EARG;
+  if (!c->tan64) {
+    set_error("currently only implemented for cases where a stiffness matrix exists");
+    return MUGPU_ERUNTIME;
+  }
+  return cell_operator64(c, d_in, d_out);
+}
+
+extern "C" int mugpu_cell_tangent_product(mugpu_cell c, const void* d_in, void* d_out) {
+  if (!c) return MUGPU_EARG;
+  if (!c->tan64) {
+    set_error("currently only implemented for cases where a stiffness matrix exists");
+    return MUGPU_ERUNTIME;
+  }
+  return run_tangent_product(c->dim, MUGPU_FP64, c->tan_kind, c->tan64, d_in, d_out, c->N);
+}
+
+extern "C" int mugpu_cell_apply_projection(mugpu_cell c, void* d_vec) {
+  if (!c) return MUGPU_EARG;
+  MUGPU_TRY(mugpu_cell_initialise(c));
+  OpSpec op;
+  op.in_mode = IN_PLAIN;
+  op.in = d_vec;
+  op.out_mode = OUT_PLAIN;
+  op.out = d_vec;
+  return run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
+}
+
+extern "C" int mugpu_cell_save_history(mugpu_cell c) {
+  if (!c) return MUGPU_EARG;
+  for (auto& m : c->mats)
+    if (m.family == MUGPU_HEP1) m.cur ^= 1;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_get_internal(mugpu_cell c, const char* name, double* h_out,
+                                       int* nb_components) {
+  if (!c || !name) return MUGPU_EARG;
+  const std::string nm(name);
+  int found = 0, ncomp = 0;
+  for (auto& m : c->mats) {
+    const double* src = nullptr;
+    int k = 0;
+    std::vector<double> host;
+    if (m.family == MUGPU_HEP1 && m.Fp[0]) {
+      if (nm == "eps_p") { src = m.ep[m.cur]; k = 1; }
+      else if (nm == "F_prev") { src = m.Fp[m.cur]; k = c->nc; }
+      else if (nm == "be") { src = m.be[m.cur]; k = c->nc; }
+      if (src) {
+        host.resize(m.h_pix.size() * k);
+        MUGPU_TRY(mugpu_download(host.data(), src, host.size() * sizeof(double)));
+      }
+    } else if (m.family == MUGPU_LE2 && nm == "eigenstrain") {
+      host = m.h_params;
+      k = c->nc;
+      src = host.data();
+    }
+    if (!k) continue;
+    if (!found) {
+      ncomp = k;
+      std::memset(h_out, 0, sizeof(double) * (size_t)c->N * k);
+    } else if (k != ncomp) {
+      set_error("The fields named '%s' do not have the same number of components in every material", name);
+      return MUGPU_ERUNTIME;
+    }
+    found = 1;
+    for (size_t q = 0; q < m.h_pix.size(); ++q)
+      std::memcpy(h_out + m.h_pix[q] * k, host.data() + q * k, sizeof(double) * k);
+  }
+  if (!found) {
+    set_error("The field named '%s' does not exist in any of the materials and can therefore not be globalised!", name);
+    return MUGPU_ERUNTIME;
+  }
+  *nb_components = ncomp;
+  return MUGPU_OK;
+}
+
+// ============================== vector kernels ================================
+extern "C" int mugpu_dot(const void* d_a, const void* d_b, int64_t n, double* h_out) {
+  MUGPU_TRY(ensure_init());
+  double* part = g_dev_scratch;
+  double* scal = g_dev_scratch + kRedBlocks;
+  MUGPU_TRY(dot_device(MUGPU_FP64, d_a, d_b, n, part, scal, FIN_STORE_TMP0));
+  MUGPU_TRY(sync_scalars(scal));
+  *h_out = g_host_scal[SC_TMP0];
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_axpby(double a, const void* d_x, double b, void* d_y, int64_t n) {
+  MUGPU_TRY(ensure_init());
+  return axpby(MUGPU_FP64, a, d_x, b, d_y, n);
+}
+
+extern "C" int mugpu_scale_copy(double a, const void* d_x, void* d_y, int64_t n) {
+  MUGPU_TRY(ensure_init());
+  return convert(MUGPU_FP64, d_x, MUGPU_FP64, d_y, n, a);
+}
+
+// ============================== SolverCG ======================================
+extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* d_r, void* d_p,
+                              void* d_Ap, double* rdr, double* rhs_norm2) {
+  if (!c) return MUGPU_EARG;
+  if (!c->tan64) {
+    set_error("currently only implemented for cases where a stiffness matrix exists");
+    return MUGPU_ERUNTIME;
+  }
+  const int64_t n = c->N * c->nc;
+  MUGPU_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, g_stream));
+  // r = A x - b, one full operator application (solver_cg.cc:62-63)
+  MUGPU_TRY(cell_operator64(c, d_x, d_Ap));
+  MUGPU_TRY(cg_init_vectors(MUGPU_FP64, d_Ap, d_rhs, d_r, d_p, n));
+  MUGPU_TRY(dot_device(MUGPU_FP64, d_r, d_r, n, c->partials, c->scal, FIN_STORE_RDR));
+  MUGPU_TRY(dot_device(MUGPU_FP64, d_rhs, d_rhs, n, c->partials, c->scal, FIN_STORE_TMP0));
+  MUGPU_TRY(sync_scalars(c->scal));
+  *rdr = g_host_scal[SC_RDR];
+  *rhs_norm2 = g_host_scal[SC_TMP0];
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cg_step(mugpu_cell c, int update_p, void* d_x, void* d_r, void* d_p,
+                             void* d_Ap, double* new_rdr) {
+  if (!c) return MUGPU_EARG;
+  const int64_t n = c->N * c->nc;
+  OpSpec op;
+  op.in_mode = update_p ? IN_TAN_PUPDATE : IN_TAN;
+  op.in = d_p;
+  op.rvec = d_r;
+  op.pout = d_p;
+  op.scal = c->scal;
+  op.tan_kind = c->tan_kind;
+  op.tangent = c->tan64;
+  op.out_mode = OUT_DOT;
+  op.out = d_Ap;
+  op.dotv = d_p;
+  MUGPU_TRY(run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum));
+  MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzblocks, c->scal, FIN_ALPHA));
+  MUGPU_TRY(cg_update(MUGPU_FP64, d_x, d_r, d_p, d_Ap, c->scal, n, c->partials));
+  MUGPU_TRY(sync_scalars(c->scal));
+  *new_rdr = g_host_scal[SC_NEWRDR];
+  return MUGPU_OK;
+}
+
+// ==================== mixed precision CG with FP64 refinement ===================
+extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, double tol,
+                                    int maxiter, double inner_tol, int64_t* iterations,
+                                    int* refinements, double* rel_residual) {
+  if (!c) return MUGPU_EARG;
+  if (c->prec != MUGPU_FP32 || !c->tan32) {
+    set_error("the mixed-precision solve needs a cell created with MUGPU_FP32 and an evaluated tangent");
+    return MUGPU_ERUNTIME;
+  }
+  const int64_t n = c->N * c->nc;
+  if (!c->x32) {
+    bool ok = hipMalloc((void**)&c->x32, sizeof(float) * n) == hipSuccess &&
+              hipMalloc((void**)&c->r32, sizeof(float) * n) == hipSuccess &&
+              hipMalloc((void**)&c->p32, sizeof(float) * n) == hipSuccess &&
+              hipMalloc((void**)&c->Ap32, sizeof(float) * n) == hipSuccess &&
+              hipMalloc((void**)&c->r64, sizeof(double) * n) == hipSuccess;
+    if (!ok) {
+      set_error("device allocation of the mixed-precision CG vectors failed");
+      return MUGPU_EMEMORY;
+    }
+  }
+  MUGPU_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, g_stream));
+  MUGPU_TRY(dot_device(MUGPU_FP64, d_rhs, d_rhs, n, c->partials, c->scal, FIN_STORE_TMP1));
+  MUGPU_TRY(sync_scalars(c->scal));
+  const double bnorm2 = g_host_scal[SC_TMP1];
+  int64_t its = 0;
+  int refs = 0;
+  double rr = bnorm2;
+  const double tol2 = tol * tol * bnorm2;
+  while (true) {
+    // true residual with the FP64 operator: r64 = b - A x
+    MUGPU_TRY(cell_operator64(c, d_x, c->r64));
+    MUGPU_TRY(axpby(MUGPU_FP64, 1.0, d_rhs, -1.0, c->r64, n));
+    MUGPU_TRY(dot_device(MUGPU_FP64, c->r64, c->r64, n, c->partials, c->scal, FIN_STORE_TMP1));
+    MUGPU_TRY(sync_scalars(c->scal));
+    rr = g_host_scal[SC_TMP1];
+    ++refs;
+    if (!(rr > tol2) || its >= maxiter || !(rr == rr)) break;
+    // inner CG in FP32 on A d = r64 / |r64|
+    const double s = 1.0 / std::sqrt(rr);
+    MUGPU_HIP(hipMemsetAsync(c->x32, 0, sizeof(float) * n, g_stream));
+    MUGPU_TRY(convert(MUGPU_FP64, c->r64, MUGPU_FP32, c->r32, n, -s));
+    MUGPU_TRY(convert(MUGPU_FP64, c->r64, MUGPU_FP32, c->p32, n, s));
+    MUGPU_TRY(dot_device(MUGPU_FP32, c->r32, c->r32, n, c->partials, c->scal, FIN_STORE_RDR));
+    MUGPU_TRY(sync_scalars(c->scal));
+    double rdr = g_host_scal[SC_RDR];
+    const double tol2i = inner_tol * inner_tol * rdr;
+    for (int i = 0; its < maxiter && (rdr > tol2i || i == 0); ++i, ++its) {
+      OpSpec op;
+      op.in_mode = i ? IN_TAN_PUPDATE : IN_TAN;
+      op.in = c->p32;
+      op.rvec = c->r32;
+      op.pout = c->p32;
+      op.scal = c->scal;
+      op.tan_kind = c->tan_kind;
+      op.tangent = c->tan32;
+      op.out_mode = OUT_DOT;
+      op.out = c->Ap32;
+      op.dotv = c->p32;
+      MUGPU_TRY(run_operator(c->plan, MUGPU_FP32, op, c->pj, c->spectrum));
+      MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzblocks, c->scal, FIN_ALPHA));
+      MUGPU_TRY(cg_update(MUGPU_FP32, c->x32, c->r32, c->p32, c->Ap32, c->scal, n, c->partials));
+      MUGPU_TRY(sync_scalars(c->scal));
+      rdr = g_host_scal[SC_NEWRDR];
+      if (!(rdr == rdr)) break;
+    }
+    MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n));
+  }
+  if (iterations) *iterations = its;
+  if (refinements) *refinements = refs;
+  if (rel_residual) *rel_residual = bnorm2 > 0 ? std::sqrt(rr / bnorm2) : 0.0;
+  if (rr > tol2 || !(rr == rr)) {
+    set_error("Conjugate gradient has not converged. After %lld steps, the mixed-precision solver FAILED with |r|/|b| = %g, cg_tol = %g",
+              (long long)its, std::sqrt(rr / bnorm2), tol);
+    return MUGPU_ECONVERGENCE;
+  }
+  return MUGPU_OK;
+}

@@ -1,0 +1,235 @@
+// Vector kernels of SolverCG (src/solver/solver_cg.cc:62-96) and of the
+// Newton drivers (src/solver/solvers.cc:142-187).  Reductions are
+// deterministic: a fixed grid writes per-workgroup partial sums, and one
+// workgroup adds them in a fixed order, so a result does not depend on
+// what else runs on the device.
+#include "mugpu_internal.hpp"
+
+namespace mugpu {
+
+namespace {
+
+constexpr int kRT = 256;
+
+template <int NT>
+__device__ double block_reduce(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  return red[0];
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(kRT) void k_dot(const A* __restrict__ a, const B* __restrict__ b,
+                                             int64_t n, double* partials) {
+  __shared__ double red[kRT];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride)
+    acc += (double)a[i] * (double)b[i];
+  const double t = block_reduce<kRT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials,
+                                                   int64_t count, double* scal, int mode) {
+  __shared__ double red[1024];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += 1024) acc += partials[i];
+  const double s = block_reduce<1024>(acc, red);
+  if (threadIdx.x == 0) {
+    switch (mode) {
+      case FIN_ALPHA:
+        scal[SC_PAP] = s;
+        scal[SC_ALPHA] = scal[SC_RDR] / s;
+        break;
+      case FIN_BETA:
+        scal[SC_NEWRDR] = s;
+        scal[SC_BETA] = s / scal[SC_RDR];
+        scal[SC_RDR] = s;
+        break;
+      case FIN_STORE_TMP0: scal[SC_TMP0] = s; break;
+      case FIN_STORE_TMP1: scal[SC_TMP1] = s; break;
+      default: scal[SC_RDR] = s; break;
+    }
+  }
+}
+
+// x += alpha p ; r += alpha Ap ; partial r.r
+template <typename T>
+__global__ __launch_bounds__(kRT) void k_cg_update(T* __restrict__ x, T* __restrict__ r,
+                                                   const T* __restrict__ p,
+                                                   const T* __restrict__ Ap,
+                                                   const double* __restrict__ scal, int64_t n,
+                                                   double* partials) {
+  __shared__ double red[kRT];
+  const double alpha = scal[SC_ALPHA];
+  const T al = (T)alpha;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    x[i] += al * p[i];
+    const T rv = r[i] + al * Ap[i];
+    r[i] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  const double t = block_reduce<kRT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// r = Ax - b ; p = -r
+template <typename T>
+__global__ __launch_bounds__(kRT) void k_cg_init(const T* __restrict__ Ax, const T* __restrict__ b,
+                                                 T* __restrict__ r, T* __restrict__ p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const T rv = Ax[i] - b[i];
+    r[i] = rv;
+    p[i] = -rv;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRT) void k_axpby(double a, const T* __restrict__ x, double b,
+                                               T* __restrict__ y, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  const T ta = (T)a, tb = (T)b;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride)
+    y[i] = ta * x[i] + tb * y[i];
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(kRT) void k_convert(const S* __restrict__ s, D* __restrict__ d,
+                                                 int64_t n, double scale) {
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride)
+    d[i] = (D)((double)s[i] * scale);
+}
+
+__global__ __launch_bounds__(kRT) void k_add_scaled(double* __restrict__ x,
+                                                    const float* __restrict__ d, double scale,
+                                                    int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride)
+    x[i] += scale * (double)d[i];
+}
+
+struct Uniform {
+  double v[9];
+};
+
+__global__ __launch_bounds__(kRT) void k_fill(double* __restrict__ F, Uniform u, int nc,
+                                              int64_t npix, int add) {
+  const int64_t n = npix * nc;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const double v = u.v[i % nc];
+    F[i] = add ? F[i] + v : v;
+  }
+}
+
+inline unsigned grid_for(int64_t n) {
+  int64_t g = (n + kRT - 1) / kRT;
+  if (g > kRedBlocks) g = kRedBlocks;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+}  // namespace
+
+int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,
+               double* d_scal, int mode) {
+  const unsigned g = grid_for(n);
+  if (prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(g), dim3(kRT), 0, stream(),
+                       (const double*)a, (const double*)b, n, d_partials);
+  else
+    hipLaunchKernelGGL((k_dot<float, float>), dim3(g), dim3(kRT), 0, stream(), (const float*)a,
+                       (const float*)b, n, d_partials);
+  MUGPU_HIP(hipGetLastError());
+  return finalize(d_partials, g, d_scal, mode);
+}
+
+int finalize(const double* d_partials, int64_t count, double* d_scal, int mode) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream(), d_partials, count, d_scal, mode);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
+              int64_t n, double* d_partials) {
+  const unsigned g = grid_for(n);
+  if (prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_cg_update<double>), dim3(g), dim3(kRT), 0, stream(), (double*)x,
+                       (double*)r, (const double*)p, (const double*)Ap, d_scal, n, d_partials);
+  else
+    hipLaunchKernelGGL((k_cg_update<float>), dim3(g), dim3(kRT), 0, stream(), (float*)x,
+                       (float*)r, (const float*)p, (const float*)Ap, d_scal, n, d_partials);
+  MUGPU_HIP(hipGetLastError());
+  return finalize(d_partials, g, const_cast<double*>(d_scal), FIN_BETA);
+}
+
+int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n) {
+  const unsigned g = grid_for(n);
+  if (prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_cg_init<double>), dim3(g), dim3(kRT), 0, stream(), (const double*)Ax,
+                       (const double*)b, (double*)r, (double*)p, n);
+  else
+    hipLaunchKernelGGL((k_cg_init<float>), dim3(g), dim3(kRT), 0, stream(), (const float*)Ax,
+                       (const float*)b, (float*)r, (float*)p, n);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int axpby(int prec, double a, const void* x, double b, void* y, int64_t n) {
+  const unsigned g = grid_for(n);
+  if (prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_axpby<double>), dim3(g), dim3(kRT), 0, stream(), a, (const double*)x, b,
+                       (double*)y, n);
+  else
+    hipLaunchKernelGGL((k_axpby<float>), dim3(g), dim3(kRT), 0, stream(), a, (const float*)x, b,
+                       (float*)y, n);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, double scale) {
+  const unsigned g = grid_for(n);
+  if (from_prec == MUGPU_FP64 && to_prec == MUGPU_FP32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(kRT), 0, stream(),
+                       (const double*)src, (float*)dst, n, scale);
+  else if (from_prec == MUGPU_FP32 && to_prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(g), dim3(kRT), 0, stream(),
+                       (const float*)src, (double*)dst, n, scale);
+  else if (from_prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(g), dim3(kRT), 0, stream(),
+                       (const double*)src, (double*)dst, n, scale);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(kRT), 0, stream(),
+                       (const float*)src, (float*)dst, n, scale);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n) {
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_add_scaled, dim3(g), dim3(kRT), 0, stream(), (double*)x64,
+                     (const float*)d32, scale, n);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add) {
+  Uniform u{};
+  for (int c = 0; c < nc && c < 9; ++c) u.v[c] = h_val[c];
+  const unsigned g = grid_for(npix * nc);
+  hipLaunchKernelGGL(k_fill, dim3(g), dim3(kRT), 0, stream(), F, u, nc, npix, add);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+}  // namespace mugpu

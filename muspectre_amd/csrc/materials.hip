@@ -1,0 +1,482 @@
+// Per-voxel constitutive kernels (FP64): one launch per material over its
+// own pixels in insertion order, as MaterialMuSpectre::compute_stresses_worker
+// iterates them (src/materials/material_muSpectre_base.hh:292-401).
+//
+//   LE1  src/materials/material_linear_elastic1.hh:166-194, .cc:40-48
+//   LE2  material_linear_elastic2.hh:178-201 (LE1 on strain - eigenstrain)
+//   LE3  material_linear_elastic3.hh:181-198 (per-pixel C from E, nu)
+//   LE4  material_linear_elastic4.hh:189-210 (per-pixel lambda, mu)
+//   HEP1 material_hyper_elasto_plastic1.hh:227-309, .cc:40-84
+//   finite strain: Green-Lagrange strain and PK2 -> PK1 (materials_toolbox.hh:
+//   139-148,313-350); Kirchhoff -> PK1 as coded (:374-410)
+//
+// The tangent is written in the representation the cell chose: ISO
+// (lambda, mu), HEP (dim^3, j-independent Kirchhoff->PK1 tangent) or FULL
+// (the reference's nc x nc matrix).
+#include <cmath>
+
+#include "mugpu_internal.hpp"
+
+namespace mugpu {
+
+namespace {
+
+template <int D>
+__device__ __forceinline__ void hooke_stress(double lam, double mu, const double* E, double* S) {
+  double tr = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) tr += E[i * (D + 1)];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) S[i + D * j] = (i == j ? tr * lam : 0.0) + 2 * mu * E[i + D * j];
+}
+
+// C : E for C = lambda Itrac + 2 mu Isymm
+template <int D>
+__device__ __forceinline__ void iso_contract(double lam, double mu, const double* E, double* S) {
+  double tr = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) tr += E[i * (D + 1)];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+      S[i + D * j] = (i == j ? lam * tr : 0.0) + mu * (E[i + D * j] + E[j + D * i]);
+}
+
+template <int D>
+__device__ __forceinline__ double iso_C(double lam, double mu, int i, int j, int k, int l) {
+  const double itrac = (i == j && k == l) ? 1.0 : 0.0;
+  const double isym = 0.5 * (((i == k && j == l) ? 1.0 : 0.0) + ((i == l && j == k) ? 1.0 : 0.0));
+  return lam * itrac + 2 * mu * isym;
+}
+
+template <int D>
+__device__ __forceinline__ void green_lagrange(const double* F, double* E) {
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += F[k + D * i] * F[k + D * j];
+      E[i + D * j] = 0.5 * (s - (i == j ? 1.0 : 0.0));
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void inverse(const double* A, double* inv) {
+  if (D == 2) {
+    const double det = A[0] * A[3] - A[2] * A[1];
+    inv[0] = A[3] / det;
+    inv[3] = A[0] / det;
+    inv[2] = -A[2] / det;
+    inv[1] = -A[1] / det;
+  } else {
+#define a(i, j) A[(i) + 3 * (j)]
+    const double c00 = a(1, 1) * a(2, 2) - a(1, 2) * a(2, 1);
+    const double c01 = a(1, 2) * a(2, 0) - a(1, 0) * a(2, 2);
+    const double c02 = a(1, 0) * a(2, 1) - a(1, 1) * a(2, 0);
+    const double det = a(0, 0) * c00 + a(0, 1) * c01 + a(0, 2) * c02;
+    const double id = 1.0 / det;
+    inv[0] = c00 * id;
+    inv[1] = c01 * id;
+    inv[2] = c02 * id;
+    inv[3] = (a(0, 2) * a(2, 1) - a(0, 1) * a(2, 2)) * id;
+    inv[4] = (a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0)) * id;
+    inv[5] = (a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1)) * id;
+    inv[6] = (a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1)) * id;
+    inv[7] = (a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2)) * id;
+    inv[8] = (a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0)) * id;
+#undef a
+  }
+}
+
+template <int D>
+__device__ __forceinline__ void matmul(const double* A, const double* B, double* C) {
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += A[i + D * k] * B[k + D * j];
+      C[i + D * j] = s;
+    }
+}
+
+// eigenvalues of a symmetric matrix, ascending (3x3: trigonometric solution
+// of the characteristic cubic, O. K. Smith, CACM 4(4), 1961)
+template <int D>
+__device__ void eig_sym(const double* A, double* ev) {
+  if (D == 2) {
+    const double a = A[0], b = 0.5 * (A[2] + A[1]), d = A[3];
+    const double m = 0.5 * (a + d), h = 0.5 * (a - d);
+    const double r = sqrt(h * h + b * b);
+    ev[0] = m - r;
+    ev[1] = m + r;
+    return;
+  }
+  const double a00 = A[0], a11 = A[4], a22 = A[8];
+  const double a01 = A[3], a02 = A[6], a12 = A[7];
+  const double m = (a00 + a11 + a22) / 3.0;
+  const double b00 = a00 - m, b11 = a11 - m, b22 = a22 - m;
+  const double p2 = b00 * b00 + b11 * b11 + b22 * b22 + 2.0 * (a01 * a01 + a02 * a02 + a12 * a12);
+  const double p = p2 / 6.0;
+  if (p <= 0.0) {
+    ev[0] = ev[1] = ev[2] = m;
+    return;
+  }
+  const double detB = b00 * (b11 * b22 - a12 * a12) - a01 * (a01 * b22 - a12 * a02) +
+                      a02 * (a01 * a12 - b11 * a02);
+  const double sp = sqrt(p);
+  double r = 0.5 * detB / (p * sp);
+  r = r > 1.0 ? 1.0 : (r < -1.0 ? -1.0 : r);
+  const double phi = acos(r) / 3.0;
+  double hi = m + 2.0 * sp * cos(phi);
+  double lo = m + 2.0 * sp * cos(phi + 2.0 * M_PI / 3.0);
+  double mid = 3.0 * m - hi - lo;
+  double e0 = lo, e1 = mid, e2 = hi, t;
+  if (e1 < e0) { t = e0; e0 = e1; e1 = t; }
+  if (e2 < e1) { t = e1; e1 = e2; e2 = t; }
+  if (e1 < e0) { t = e0; e0 = e1; e1 = t; }
+  ev[0] = e0;
+  ev[1] = e1;
+  ev[2] = e2;
+}
+
+__device__ __forceinline__ double log_dd1(double a, double b) {
+  const double h = b - a;
+  return h == 0.0 ? 1.0 / a : log1p(h / a) / h;
+}
+__device__ __forceinline__ double exp_dd1(double a, double b) {
+  const double h = b - a;
+  return h == 0.0 ? exp(a) : exp(a) * expm1(h) / h;
+}
+
+// f(T) = d0 I + d1 (T - l0 I) + d2 (T - l0 I)(T - l1 I): the Newton form of
+// Sylvester's interpolation over the eigenvalues (same function in exact
+// arithmetic as src/common/eigen_tools.hh:221-341, finite at repeated
+// eigenvalues)
+template <int D>
+__device__ void newton_poly(const double* T_, const double* l, double d0, double d1, double d2,
+                            double* out) {
+  double A[D * D], B[D * D];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      A[i + D * j] = T_[i + D * j] - (i == j ? l[0] : 0.0);
+      B[i + D * j] = T_[i + D * j] - (i == j ? l[1] : 0.0);
+    }
+  double AB[D * D];
+  matmul<D>(A, B, AB);
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i)
+      out[i + D * j] = (i == j ? d0 : 0.0) + d1 * A[i + D * j] + (D == 3 ? d2 * AB[i + D * j] : 0.0);
+}
+
+template <int D>
+__device__ void logm(const double* T_, double* out) {
+  double l[3];
+  eig_sym<D>(T_, l);
+  double d2 = 0.0;
+  if (D == 3) {
+    const double h = l[2] - l[0];
+    d2 = h == 0.0 ? -0.5 / (l[0] * l[0]) : (log_dd1(l[1], l[2]) - log_dd1(l[0], l[1])) / h;
+  }
+  newton_poly<D>(T_, l, log(l[0]), log_dd1(l[0], l[1]), d2, out);
+}
+
+template <int D>
+__device__ void expm(const double* T_, double* out) {
+  double l[3];
+  eig_sym<D>(T_, l);
+  double d2 = 0.0;
+  if (D == 3) {
+    const double h = l[2] - l[0];
+    d2 = h == 0.0 ? 0.5 * exp(l[0]) : (exp_dd1(l[1], l[2]) - exp_dd1(l[0], l[1])) / h;
+  }
+  newton_poly<D>(T_, l, exp(l[0]), exp_dd1(l[0], l[1]), d2, out);
+}
+
+struct MatArgs {
+  const int* pix;
+  int64_t n;
+  double young, poisson, lambda, mu, bulk, tau_y0, H;
+  const double* params;
+  const double* F;
+  double* P;
+  double* tan;
+  int tan_kind;
+  int need_tangent;
+  double *Fp_cur, *be_cur, *ep_cur;
+  const double *Fp_old, *be_old, *ep_old;
+};
+
+// write an isotropic tangent (lambda, mu) in the cell's representation
+template <int D>
+__device__ void store_iso(const MatArgs& a, int64_t v, double lam, double mu) {
+  constexpr int NC = D * D;
+  if (a.tan_kind == TAN_ISO) {
+    a.tan[2 * v] = lam;
+    a.tan[2 * v + 1] = mu;
+  } else {
+    double* K = a.tan + v * NC * NC;
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+#pragma unroll
+          for (int i = 0; i < D; ++i) K[(i + D * j) + NC * (k + D * l)] = iso_C<D>(lam, mu, i, j, k, l);
+  }
+}
+
+// PK2 -> PK1 with C = lambda Itrac + 2 mu Isymm (materials_toolbox.hh:332-349)
+template <int D>
+__device__ void pk1_from_pk2_store(const MatArgs& a, int64_t v, const double* F, const double* S,
+                                   double lam, double mu) {
+  constexpr int NC = D * D;
+  double* P = a.P + v * NC;
+  double Pl[NC];
+  matmul<D>(F, S, Pl);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) P[q] = Pl[q];
+  if (!a.need_tangent) return;
+  double* K = a.tan + v * NC * NC;
+  for (int n = 0; n < D; ++n)
+    for (int j = 0; j < D; ++j)
+      for (int m = 0; m < D; ++m)
+        for (int i = 0; i < D; ++i) {
+          double k = (i == j) ? S[m + D * n] : 0.0;
+          for (int r = 0; r < D; ++r)
+            for (int s = 0; s < D; ++s) k += F[i + D * r] * iso_C<D>(lam, mu, r, m, n, s) * F[j + D * s];
+          K[(i + D * m) + NC * (j + D * n)] = k;
+        }
+}
+
+template <int D, int FAM, int FORM>
+__global__ __launch_bounds__(256) void k_linear_elastic(MatArgs a) {
+  constexpr int NC = D * D;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const int64_t v = a.pix[q];
+  double F[NC], E[NC], S[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) F[c] = a.F[v * NC + c];
+  double lam = a.lambda, mu = a.mu;
+  if (FAM == MUGPU_LE3 || FAM == MUGPU_LE4) {
+    lam = a.params[2 * q];
+    mu = a.params[2 * q + 1];
+  }
+  if (FORM == MUGPU_SMALL_STRAIN) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) E[c] = F[c];
+  } else {
+    green_lagrange<D>(F, E);
+  }
+  if (FAM == MUGPU_LE2) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) E[c] -= a.params[q * NC + c];
+  }
+  if (FAM == MUGPU_LE1 || FAM == MUGPU_LE2) hooke_stress<D>(lam, mu, E, S);
+  else iso_contract<D>(lam, mu, E, S);
+  if (FORM == MUGPU_SMALL_STRAIN) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) a.P[v * NC + c] = S[c];
+    if (a.need_tangent) store_iso<D>(a, v, lam, mu);
+  } else {
+    pk1_from_pk2_store<D>(a, v, F, S, lam, mu);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
+  constexpr int NC = D * D;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= a.n) return;
+  const int64_t v = a.pix[q];
+  const double lam = a.lambda, mu = a.mu;
+  double F[NC], Fpo[NC], beo[NC], Fpi[NC], f[NC], tmp[NC], bes[NC], lnb[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    F[c] = a.F[v * NC + c];
+    Fpo[c] = a.Fp_old[q * NC + c];
+    beo[c] = a.be_old[q * NC + c];
+  }
+  const double eps_old = a.ep_old[q];
+  // trial state: f = F Fprev^-1, be* = f be f^T, ln be*
+  inverse<D>(Fpo, Fpi);
+  matmul<D>(F, Fpi, f);
+  matmul<D>(f, beo, tmp);
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += tmp[i + D * k] * f[j + D * k];
+      bes[i + D * j] = s;
+    }
+  logm<D>(bes, lnb);
+  double ts[NC], td[NC], Ns[NC], tau[NC];
+  hooke_stress<D>(lam, mu, lnb, ts);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) ts[c] *= 0.5;
+  double tr = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) tr += ts[i * (D + 1)];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) td[i + D * j] = ts[i + D * j] - (i == j ? tr / D : 0.0);
+  double ss = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) ss += td[i + D * j] * td[j + D * i];
+  const double taueq = sqrt(3 * .5 * ss);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) Ns[c] = 3 * .5 * td[c] / taueq;
+  double phi = taueq - a.tau_y0 - a.H * eps_old;
+  phi = (phi < 0.0) ? 0.0 : phi;
+  const double dg = phi / (a.H + 3 * mu);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) tau[c] = ts[c] - 2 * dg * mu * Ns[c];
+  // internal variable updates, as coded (eps_p.current() += dgamma)
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    a.Fp_cur[q * NC + c] = F[c];
+    lnb[c] -= 2 * dg * Ns[c];
+  }
+  double be_new[NC];
+  expm<D>(lnb, be_new);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) a.be_cur[q * NC + c] = be_new[c];
+  a.ep_cur[q] += dg;
+  // P = tau F^-T
+  double Fi[NC];
+  inverse<D>(F, Fi);
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      double s = 0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) s += tau[i + D * k] * Fi[j + D * k];
+      a.P[v * NC + i + D * j] = s;
+    }
+  if (!a.need_tangent) return;
+  // Kirchhoff tangent C(r,m,n,s), then M(i,m,n) = sum_r Finv(i,r) sum_s C(r,m,n,s)
+  const bool plastic = phi > 0;
+  const double a0 = plastic ? dg * mu / taueq : 0.0;
+  const double a1 = mu / (a.H + 3 * mu);
+  double csum[D * D * D];  // c(r,m,n) = sum_s C(r,m,n,s) at r + D m + D^2 n
+#pragma unroll
+  for (int n = 0; n < D; ++n)
+#pragma unroll
+    for (int m = 0; m < D; ++m)
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        double acc = 0;
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+          double C;
+          if (plastic) {
+            const double itrac = (r == m && n == s) ? 1.0 : 0.0;
+            const double isym = 0.5 * (((r == n && m == s) ? 1.0 : 0.0) + ((r == s && m == n) ? 1.0 : 0.0));
+            C = (a.bulk / 2. - mu / 3 + a0 * mu) * itrac + (1 - 3 * a0) * mu * isym +
+                2 * mu * (a0 - a1) * Ns[r + D * m] * Ns[n + D * s];
+          } else {
+            C = 0.5 * iso_C<D>(lam, mu, r, m, n, s);
+          }
+          acc += C;
+        }
+        csum[r + D * m + D * D * n] = acc;
+      }
+  double M[D * D * D];
+#pragma unroll
+  for (int n = 0; n < D; ++n)
+#pragma unroll
+    for (int m = 0; m < D; ++m)
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+        double acc = 0;
+#pragma unroll
+        for (int r = 0; r < D; ++r) acc += Fi[i + D * r] * csum[r + D * m + D * D * n];
+        M[i + D * m + D * D * n] = acc;
+      }
+  if (a.tan_kind == TAN_HEP) {
+    double* out = a.tan + v * (D * D * D);
+#pragma unroll
+    for (int e = 0; e < D * D * D; ++e) out[e] = M[e];
+  } else {
+    double* K = a.tan + v * NC * NC;
+    for (int n = 0; n < D; ++n)
+      for (int j = 0; j < D; ++j)
+        for (int m = 0; m < D; ++m)
+          for (int i = 0; i < D; ++i) K[(i + D * m) + NC * (j + D * n)] = M[i + D * m + D * D * n];
+  }
+}
+
+template <int D, int FORM>
+int launch_le(const MaterialDev& m, MatArgs& a, dim3 grid) {
+  switch (m.family) {
+    case MUGPU_LE1: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE1, FORM>), grid, dim3(256), 0, stream(), a); break;
+    case MUGPU_LE2: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE2, FORM>), grid, dim3(256), 0, stream(), a); break;
+    case MUGPU_LE3: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE3, FORM>), grid, dim3(256), 0, stream(), a); break;
+    default: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE4, FORM>), grid, dim3(256), 0, stream(), a); break;
+  }
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+}  // namespace
+
+int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,
+                  int tan_kind, double* tangent, int need_tangent) {
+  const int64_t n = (int64_t)m.h_pix.size();
+  if (n == 0) return MUGPU_OK;
+  MatArgs a{};
+  a.pix = m.d_pix;
+  a.n = n;
+  a.young = m.young;
+  a.poisson = m.poisson;
+  a.lambda = m.lambda;
+  a.mu = m.mu;
+  a.bulk = m.bulk;
+  a.tau_y0 = m.tau_y0;
+  a.H = m.H;
+  a.params = m.d_params;
+  a.F = F;
+  a.P = P;
+  a.tan = tangent;
+  a.tan_kind = tan_kind;
+  a.need_tangent = need_tangent;
+  dim3 grid((unsigned)((n + 255) / 256));
+  if (m.family == MUGPU_HEP1) {
+    const int cur = m.cur, old = cur ^ 1;
+    a.Fp_cur = m.Fp[cur];
+    a.be_cur = m.be[cur];
+    a.ep_cur = m.ep[cur];
+    a.Fp_old = m.Fp[old];
+    a.be_old = m.be[old];
+    a.ep_old = m.ep[old];
+    if (dim == 3) hipLaunchKernelGGL((k_hep1<3>), grid, dim3(256), 0, stream(), a);
+    else hipLaunchKernelGGL((k_hep1<2>), grid, dim3(256), 0, stream(), a);
+    MUGPU_HIP(hipGetLastError());
+    return MUGPU_OK;
+  }
+  if (dim == 3)
+    return form == MUGPU_SMALL_STRAIN ? launch_le<3, MUGPU_SMALL_STRAIN>(m, a, grid)
+                                      : launch_le<3, MUGPU_FINITE_STRAIN>(m, a, grid);
+  return form == MUGPU_SMALL_STRAIN ? launch_le<2, MUGPU_SMALL_STRAIN>(m, a, grid)
+                                    : launch_le<2, MUGPU_FINITE_STRAIN>(m, a, grid);
+}
+
+}  // namespace mugpu

@@ -1,0 +1,137 @@
+// Host-side structures and kernel launchers shared by the HIP translation
+// units of libmugpu.so.  Nothing here is part of the public C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fft_core.hpp"
+#include "mugpu.h"
+
+namespace mugpu {
+
+// ---- error handling --------------------------------------------------------
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+#define MUGPU_HIP(call)                                                   \
+  do {                                                                    \
+    hipError_t e_ = (call);                                               \
+    if (e_ != hipSuccess) return ::mugpu::hip_fail(e_, #call, __FILE__, __LINE__); \
+  } while (0)
+#define MUGPU_TRY(call)          \
+  do {                           \
+    int rc_ = (call);            \
+    if (rc_ != MUGPU_OK) return rc_; \
+  } while (0)
+
+hipStream_t stream();
+int max_lds_per_block();
+
+// ---- tangent representations of the directional-stiffness product --------
+// FULL: the reference's stored K, nc*nc per voxel, K(r,c) at r + nc*c
+//       (src/common/T4_map_proxy.hh:83-105)
+// ISO : (lambda, mu) per voxel; dP = lambda tr(dF) I + mu (dF + dF^T), the
+//       contraction with C = lambda Itrac + 2 mu Isymm
+// HEP : dim^3 per voxel, M(i,m,n) at i + d m + d^2 n, for the j-independent
+//       Kirchhoff->PK1 tangent K(i,m,j,n) = M(i,m,n)
+//       (src/materials/materials_toolbox.hh:394-406)
+enum TangentKind { TAN_FULL = 0, TAN_ISO = 1, TAN_HEP = 2 };
+inline int tangent_stride(int kind, int dim) {
+  const int nc = dim * dim;
+  return kind == TAN_FULL ? nc * nc : kind == TAN_ISO ? 2 : dim * dim * dim;
+}
+
+// device scalar block used by the CG kernels
+enum Scalar { SC_RDR = 0, SC_PAP, SC_ALPHA, SC_NEWRDR, SC_BETA, SC_TMP0, SC_TMP1, SC_COUNT };
+
+// ---- FFT plan -----------------------------------------------------------------
+struct FftPlan {
+  int dim = 0, nc = 0, prec = MUGPU_FP64;
+  int res[3] = {1, 1, 1}, hres[3] = {1, 1, 1};
+  int64_t N = 0, kpts = 0;
+  AxisPlan ax[3];
+  void* tw64[3] = {nullptr, nullptr, nullptr};  // double2 tables W_n^m
+  void* tw32[3] = {nullptr, nullptr, nullptr};  // float2 tables
+  // z-pass: rows per workgroup; strided passes: lines per workgroup
+  int zrb = 1;
+  int yW = 1;
+  int xW = 1;      // plain axis-0 pass
+  int xWk = 1;     // fused axis-0 projection pass: k-points per workgroup
+  int xNT = 256;   // threads of the fused pass
+  void* partials = nullptr;  // double per z-pass workgroup
+  int64_t nzblocks = 0;
+};
+
+int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec);
+void plan_free(FftPlan& p);
+
+// projection parameters handed to the fused axis-0 kernel
+struct ProjParams {
+  int kind = -1;  // MUGPU_PROJ_*; -1 none
+  int allow_even = 0;
+  double L[3] = {1, 1, 1};
+  double norm = 1.0;
+};
+
+// input stage of the last-axis r2c pass
+enum InMode { IN_PLAIN = 0, IN_TAN = 1, IN_TAN_PUPDATE = 2 };
+// output stage of the last-axis c2r pass
+enum OutMode { OUT_PLAIN = 0, OUT_DOT = 1 };
+
+struct OpSpec {
+  int in_mode = IN_PLAIN;
+  const void* in = nullptr;      // real field (plan precision)
+  const void* rvec = nullptr;    // CG residual for IN_TAN_PUPDATE
+  void* pout = nullptr;          // updated p for IN_TAN_PUPDATE
+  const double* scal = nullptr;  // device scalars (SC_BETA)
+  int tan_kind = TAN_FULL;
+  const void* tangent = nullptr; // plan precision
+  int out_mode = OUT_PLAIN;
+  void* out = nullptr;           // real field (plan precision)
+  const void* dotv = nullptr;    // vector dotted with the output for OUT_DOT
+};
+
+// full operator: r2c (with input stage) -> [axis passes] -> projection ->
+// [inverse axis passes] -> c2r (with output stage); spectrum is workspace
+int run_operator(FftPlan& p, int prec, const OpSpec& op, const ProjParams& pj,
+                 void* spectrum);
+// engine transforms (no projection)
+int run_r2c(FftPlan& p, int prec, const void* in, void* spectrum);
+int run_c2r(FftPlan& p, int prec, void* spectrum, void* out);
+
+// ---- reductions and vector kernels (cg.hip) --------------------------------
+int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,
+               double* d_scal, int slot);
+int finalize(const double* d_partials, int64_t count, double* d_scal, int mode);
+enum FinalizeMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_STORE_TMP0 = 2, FIN_STORE_TMP1 = 3,
+                    FIN_STORE_RDR = 4 };
+int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
+              int64_t n, double* d_partials);
+int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n);
+int axpby(int prec, double a, const void* x, double b, void* y, int64_t n);
+int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, double scale);
+int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n);
+int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add);
+constexpr int kRedBlocks = 1024;
+
+// ---- materials (materials.hip) ----------------------------------------------
+struct MaterialDev {
+  int family = 0;
+  double young = 0, poisson = 0, lambda = 0, mu = 0, bulk = 0, tau_y0 = 0, H = 0;
+  std::vector<int64_t> h_pix;
+  std::vector<double> h_params;  // per pixel
+  int ppn = 0;
+  int* d_pix = nullptr;
+  double* d_params = nullptr;
+  double* Fp[2] = {nullptr, nullptr};
+  double* be[2] = {nullptr, nullptr};
+  double* ep[2] = {nullptr, nullptr};
+  int cur = 0;
+};
+
+int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,
+                  int tan_kind, double* tangent, int need_tangent);
+
+}  // namespace mugpu

@@ -1,0 +1,175 @@
+"""GPU parity of the device cell (materials, directional stiffness, CG) against
+the oracle, called through the C-ABI.
+
+Reference behaviour: CellBase::evaluate_stress_tangent and
+evaluate_projected_directional_stiffness (src/cell/cell_base.cc:129-192),
+SolverCG::solve (src/solver/solver_cg.cc:54-110).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from muspectre_amd._device import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+DP = ctypes.POINTER(ctypes.c_double)
+
+
+def _arr(a):
+    return a.ctypes.data_as(DP)
+
+
+class GpuCell:
+    def __init__(self, capi, res, lengths, form, prec=0, allow_even=0):
+        self.capi = capi
+        self.lib = capi.load()
+        self.res = list(res)
+        self.d = len(res)
+        self.nc = self.d ** 2
+        self.N = int(np.prod(res))
+        r = (ctypes.c_int * self.d)(*res)
+        L = (ctypes.c_double * self.d)(*lengths)
+        h = ctypes.c_void_p()
+        capi.check(self.lib.mugpu_cell_create(self.d, r, L, form, prec, allow_even, ctypes.byref(h)))
+        self.h = h
+
+    def add_material(self, fam, E=0.0, nu=0.0, tau=0.0, H=0.0):
+        m = ctypes.c_int()
+        self.capi.check(self.lib.mugpu_cell_add_material(self.h, fam, E, nu, tau, H, ctypes.byref(m)))
+        return m.value
+
+    def add_pixels(self, m, pix, params=None):
+        pix = np.ascontiguousarray(pix, dtype=np.int64)
+        pp = None if params is None else np.ascontiguousarray(params, dtype=np.float64)
+        self.capi.check(self.lib.mugpu_material_add_pixels(
+            self.h, m, len(pix), pix.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            None if pp is None else _arr(pp)))
+
+    def ptr(self, which):
+        p = ctypes.c_void_p()
+        fn = self.lib.mugpu_cell_strain if which == "F" else self.lib.mugpu_cell_stress
+        self.capi.check(fn(self.h, ctypes.byref(p)))
+        return p
+
+    def set_F(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        self.capi.check(self.lib.mugpu_upload(self.ptr("F"), F.ctypes.data_as(ctypes.c_void_p), F.nbytes))
+
+    def get(self, which):
+        out = np.empty((self.N, self.nc))
+        self.capi.check(self.lib.mugpu_download(out.ctypes.data_as(ctypes.c_void_p), self.ptr(which), out.nbytes))
+        return out
+
+    def __del__(self):
+        self.lib.mugpu_cell_destroy(self.h)
+
+
+def _two_phase(oc, gc, mask, fam_in, fam_out, Ein, Eout, nu, extra=None):
+    mi_o = oc.add_material(fam_in, Ein, nu, *(extra or (0.0, 0.0)))
+    mo_o = oc.add_material(fam_out, Eout, nu, *(extra or (0.0, 0.0)))
+    mi_g = gc.add_material(fam_in, Ein, nu, *(extra or (0.0, 0.0)))
+    mo_g = gc.add_material(fam_out, Eout, nu, *(extra or (0.0, 0.0)))
+    idx = np.arange(mask.size)
+    for i in idx:
+        oc.add_pixel(mi_o if mask.flat[i] else mo_o, int(i))
+    gc.add_pixels(mi_g, idx[mask.ravel()])
+    gc.add_pixels(mo_g, idx[~mask.ravel()])
+
+
+def _sphere_mask(res, r):
+    g = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
+    c = [n // 2 for n in res]
+    return sum((gi - ci) ** 2 for gi, ci in zip(g, c)) < r * r
+
+
+CASES = [
+    ("2d_small_le1", [51, 51], [7., 5.], 1, 1, 1, 10e9, 70e9),
+    ("3d_small_le1", [9, 7, 11], [1., 1., 1.], 1, 1, 1, 100e9, 10e9),
+    ("2d_finite_le1", [7, 9], [1., 1.], 0, 1, 1, 210e9, 70e9),
+    ("3d_finite_le1", [5, 7, 5], [1., 1., 1.], 0, 1, 1, 210e9, 70e9),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_stress_tangent_and_directional_stiffness(capi, oracle, case):
+    name, res, L, form, f_in, f_out, Ein, Eout = case
+    d = len(res)
+    nc = d * d
+    oc = oracle.Cell(res, L, form)
+    gc = GpuCell(capi, res, L, form)
+    _two_phase(oc, gc, _sphere_mask(res, min(res) / 3), f_in, f_out, Ein, Eout, .33)
+    oc.initialise()
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    rng = np.random.default_rng(11)
+    F = 0.01 * rng.standard_normal((gc.N, nc))
+    if form == 0:
+        F += np.eye(d).ravel(order="F")
+    oc.F[:] = F
+    gc.set_F(F)
+    oc.evaluate_stress_tangent()
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    P = gc.get("P")
+    assert np.abs(P - oc.P).max() <= 1e-12 * np.abs(oc.P).max()
+    K = np.zeros((gc.N, nc * nc))
+    capi.check(gc.lib.mugpu_cell_get_tangent(gc.h, _arr(K)))
+    assert np.abs(K - oc.K).max() <= 1e-12 * np.abs(oc.K).max()
+    dF = rng.standard_normal((gc.N, nc))
+    ref = oc.directional_stiffness(dF)
+    bi = DeviceBuffer.from_array(dF)
+    bo = DeviceBuffer(dF.nbytes)
+    capi.check(gc.lib.mugpu_cell_directional_stiffness(gc.h, bi.ptr, bo.ptr))
+    out = bo.download(dF.shape)
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _gpu_cg(gc, rhs, tol, maxiter):
+    """Host loop of SolverCG::solve (src/solver/solver_cg.cc:54-110) over the
+    device kernels."""
+    lib, capi = gc.lib, gc.capi
+    n = gc.N * gc.nc
+    b = DeviceBuffer.from_array(rhs)
+    x, r, p, Ap = (DeviceBuffer(8 * n) for _ in range(4))
+    rdr = ctypes.c_double()
+    bn2 = ctypes.c_double()
+    capi.check(lib.mugpu_cg_start(gc.h, b.ptr, x.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(rdr), ctypes.byref(bn2)))
+    tol2 = tol * tol * bn2.value
+    rd = rdr.value
+    it = 0
+    while it < maxiter and (rd > tol2 or it == 0):
+        nr = ctypes.c_double()
+        capi.check(lib.mugpu_cg_step(gc.h, 1 if it > 0 else 0, x.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(nr)))
+        rd = nr.value
+        it += 1
+    return x.download((gc.N, gc.nc)), it, rd < tol2
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
+def test_cg_solve_vs_oracle(capi, oracle, case):
+    name, res, L, form, f_in, f_out, Ein, Eout = case
+    d = len(res)
+    nc = d * d
+    oc = oracle.Cell(res, L, form)
+    gc = GpuCell(capi, res, L, form)
+    _two_phase(oc, gc, _sphere_mask(res, min(res) / 3), f_in, f_out, Ein, Eout, .3)
+    oc.initialise()
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    F = np.zeros((gc.N, nc))
+    if form == 0:
+        F += np.eye(d).ravel(order="F")
+    macro = np.zeros((d, d))
+    macro[0, 1] = macro[1, 0] = 0.01
+    F += macro.ravel(order="F")
+    oc.F[:] = F
+    gc.set_F(F)
+    oc.evaluate_stress_tangent()
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    rhs = -oc.apply_projection(oc.P.copy())
+    start = oc.counter.value
+    x_ref = oc.cg_solve(rhs, 1e-8, 400)
+    it_ref = oc.counter.value - start
+    x, it, conv = _gpu_cg(gc, rhs, 1e-8, 400)
+    assert conv
+    assert it == it_ref
+    assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)

@@ -1,0 +1,142 @@
+"""GPU parity of the HIP FFT engine and projections, called through the C-ABI.
+
+Reference behaviour: FFTWEngine::fft / ifft (src/fft/fftw_engine.cc:130-158),
+tested in the reference by tests/python_fft_tests.py:60-101 (numpy rfftn /
+irfftn, tolerance 1e-14 * N) and test_fftw_engine.cc:91-135; projections by
+projection_*.cc against the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from muspectre_amd._device import DeviceBuffer
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [[6, 4], [4, 6], [5, 5], [3, 3], [51, 51], [17, 4], [3, 5, 7], [7, 7, 7],
+         [16, 16, 16], [8, 6, 4], [9, 11, 13], [32, 32, 32], [12, 10, 8],
+         [125, 5, 3], [2, 2, 2], [1, 10], [64, 48, 40]]
+
+
+def _fft_plan(capi, res, prec):
+    lib = capi.load()
+    r = (ctypes.c_int * len(res))(*res)
+    h = ctypes.c_void_p()
+    capi.check(lib.mugpu_fft_create(len(res), r, len(res) ** 2, prec, ctypes.byref(h)))
+    return h
+
+
+@pytest.mark.parametrize("res", SIZES, ids=lambda r: "x".join(map(str, r)))
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_fft_r2c_c2r_vs_numpy(capi, res, prec):
+    lib = capi.load()
+    d = len(res)
+    nc = d * d
+    rng = np.random.default_rng(abs(hash(tuple(res))) % 2 ** 31)
+    dt = np.float64 if prec == 0 else np.float32
+    ct = np.complex128 if prec == 0 else np.complex64
+    x = rng.random([*res, d, d])
+    h = _fft_plan(capi, res, prec)
+    try:
+        kp = lib.mugpu_fft_nb_fourier_points(h)
+        herm = list(res[:-1]) + [res[-1] // 2 + 1]
+        assert kp == int(np.prod(herm))
+        din = DeviceBuffer.from_array(x.reshape(-1, nc).astype(dt))
+        dspec = DeviceBuffer(kp * nc * np.dtype(ct).itemsize)
+        capi.check(lib.mugpu_fft_r2c(h, din.ptr, dspec.ptr))
+        spec = dspec.download((kp, nc), ct)
+        ref = np.fft.rfftn(x, axes=tuple(range(d))).reshape(kp, nc)
+        if prec == 0:
+            # the reference's criterion: |err| < 1e-14 * N (python_fft_tests.py:66)
+            assert np.linalg.norm(spec - ref) < 1e-14 * np.prod(res)
+        else:
+            assert np.abs(spec - ref).max() < 2e-6 * np.abs(ref).max()
+        # input preserved (FFTW_PRESERVE_INPUT, fftw_engine.cc:98-100)
+        np.testing.assert_array_equal(din.download(x.reshape(-1, nc).shape, dt),
+                                      x.reshape(-1, nc).astype(dt))
+        # c2r of a random Hermitian-half spectrum vs numpy irfftn (python_fft_tests.py:85-101)
+        c = rng.random([*herm, d, d]) + 1j * rng.random([*herm, d, d])
+        dspec.upload(c.reshape(kp, nc).astype(ct))
+        dout = DeviceBuffer(int(np.prod(res)) * nc * np.dtype(dt).itemsize)
+        capi.check(lib.mugpu_fft_c2r(h, dspec.ptr, dout.ptr))
+        out = dout.download((int(np.prod(res)), nc), dt) * lib.mugpu_fft_normalisation(h)
+        ref2 = np.fft.irfftn(c, s=res, axes=tuple(range(d))).reshape(-1, nc)
+        if prec == 0:
+            assert np.linalg.norm(out - ref2) < 1e-14 * np.prod(res)
+        else:
+            assert np.abs(out - ref2).max() < 2e-6 * np.abs(ref2).max()
+    finally:
+        lib.mugpu_fft_destroy(h)
+
+
+def _projection(capi, res, lengths, kind, allow_even=0, prec=0):
+    lib = capi.load()
+    h = _fft_plan(capi, res, prec)
+    L = (ctypes.c_double * len(res))(*lengths)
+    p = ctypes.c_void_p()
+    rc = lib.mugpu_projection_create(h, kind, L, allow_even, ctypes.byref(p))
+    if rc != 0:
+        lib.mugpu_fft_destroy(h)
+        capi.check(rc)
+    return p
+
+
+PROJ_CASES = [([5, 5], [5., 5.]), ([3, 5], [1.3, 2.1]), ([7, 7, 7], [7., 7., 7.]),
+              ([3, 5, 7], [1., 2., 3.]), ([51, 51], [7., 5.]), ([9, 7, 5], [1., 1., 1.]),
+              ([25, 27, 15], [2.5, 1., 3.])]
+
+
+@pytest.mark.parametrize("res,lengths", PROJ_CASES, ids=lambda v: "x".join(map(str, v)))
+@pytest.mark.parametrize("kind", [0, 1, 2], ids=["small", "finite_fast", "finite"])
+def test_projection_vs_oracle(capi, oracle, res, lengths, kind):
+    lib = capi.load()
+    d = len(res)
+    nc = d * d
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal((int(np.prod(res)), nc))
+    ref = oracle.apply_projection(kind, res, lengths, f)
+    p = _projection(capi, res, lengths, kind)
+    try:
+        buf = DeviceBuffer.from_array(f)
+        capi.check(lib.mugpu_projection_apply(p, buf.ptr))
+        out = buf.download(f.shape)
+        assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+        # get_operator equals the oracle's stored operator
+        kp = int(np.prod(list(res[:-1]) + [res[-1] // 2 + 1]))
+        nop = d if kind == 1 else nc * nc
+        op = np.zeros((kp, nop))
+        capi.check(lib.mugpu_projection_get_operator(p, op.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        np.testing.assert_allclose(op, oracle.projection_operator(kind, res, lengths),
+                                   rtol=0, atol=1e-14)
+    finally:
+        lib.mugpu_projection_destroy(p)
+
+
+def test_projection_rejects_even_grid(capi):
+    """ProjectionError for even grids (tests/test_projection_finite.cc:71-77)."""
+    with pytest.raises(capi.MugpuError) as e:
+        _projection(capi, [4, 5], [1., 1.], 1)
+    assert e.value.code == capi.EPROJECTION
+
+
+@pytest.mark.parametrize("res", [[8, 8], [16, 12, 10], [64, 64, 64]],
+                         ids=lambda r: "x".join(map(str, r)))
+@pytest.mark.parametrize("kind", [0, 1])
+def test_projection_even_policy_vs_oracle(capi, oracle, res, kind):
+    lib = capi.load()
+    d = len(res)
+    f = np.random.default_rng(3).standard_normal((int(np.prod(res)), d * d))
+    ref = oracle.apply_projection(kind, res, [1.0] * d, f, allow_even=True)
+    p = _projection(capi, res, [1.0] * d, kind, allow_even=1)
+    try:
+        buf = DeviceBuffer.from_array(f)
+        capi.check(lib.mugpu_projection_apply(p, buf.ptr))
+        out = buf.download(f.shape)
+        assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+        # idempotent: G(G f) = G f
+        capi.check(lib.mugpu_projection_apply(p, buf.ptr))
+        out2 = buf.download(f.shape)
+        assert np.abs(out2 - out).max() <= 1e-13 * np.abs(out).max()
+    finally:
+        lib.mugpu_projection_destroy(p)
