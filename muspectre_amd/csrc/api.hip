@@ -677,7 +677,13 @@ extern "C" int mugpu_cell_save_history(mugpu_cell c) {
 extern "C" int mugpu_cell_get_internal(mugpu_cell c, const char* name, double* h_out,
                                        int* nb_components) {
   if (!c || !name) return MUGPU_EARG;
-  const std::string nm(name);
+  // short names, or the reference's field names (material_linear_elastic2.cc:59,
+  // material_hyper_elasto_plastic1.cc:44-52)
+  std::string nm(name);
+  if (nm == "Eigenstrain") nm = "eigenstrain";
+  if (nm == "cumulated plastic flow \xce\xb5\xe2\x82\x9a") nm = "eps_p";
+  if (nm == "Previous placement gradient F\xe1\xb5\x97") nm = "F_prev";
+  if (nm == "Previous left Cauchy-Green deformation b\xe2\x82\x91\xe1\xb5\x97") nm = "be";
   int found = 0, ncomp = 0;
   for (auto& m : c->mats) {
     const double* src = nullptr;

@@ -1,0 +1,272 @@
+"""The reference's Python-level tests, run against this build's _muSpectre
+(the HIP path), plus full newton_cg / de_geus parity against the oracle.
+
+Sources of the checks: tests/python_binding_tests.py:47-194,
+python_fft_tests.py:53-101, python_material_linear_elastic3_test.py:44-85,
+python_material_linear_elastic4_test.py:44-86, test_solver_newton_cg.cc:
+55-449 (reference repository).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def µ():
+    import muspectre_amd
+    return muspectre_amd
+
+
+# ---- python_fft_tests.py:53-101 ------------------------------------------
+@pytest.mark.parametrize("res", [[6, 4], [7, 5], [4, 6, 5], [5, 5, 5]])
+def test_fft_forward_reverse(µ, res):
+    dim = len(res)
+    tol = 1e-14 * np.prod(res)
+    engine = µ.fft.FFT(res, dim ** 2)
+    engine.initialise()
+    in_arr = np.random.random([*res, dim, dim])
+    out_ref = np.fft.rfftn(in_arr, axes=tuple(range(dim)))
+    out_msp = engine.fft(in_arr.reshape(-1, dim ** 2).T).T
+    assert np.linalg.norm(out_ref - out_msp.reshape(out_ref.shape)) < tol
+    complex_res = µ.get_hermitian_sizes(res)
+    c = np.zeros([*complex_res, dim, dim], dtype=complex)
+    c.real = np.random.random(c.shape)
+    c.imag = np.random.random(c.shape)
+    ref = np.fft.irfftn(c, s=res, axes=tuple(range(dim)))
+    out = engine.ifft(c.reshape(-1, dim ** 2).T).T * engine.normalisation()
+    assert np.linalg.norm(ref - out.reshape(ref.shape)) < tol
+
+
+def test_fft_requires_initialise(µ):
+    engine = µ.fft.FFT([5, 5], 4)
+    with pytest.raises(RuntimeError):
+        engine.fft(np.zeros((4, 25)))
+
+
+# ---- python_binding_tests.py:47-91 ---------------------------------------
+def test_cell_construction_and_add_pixel(µ):
+    sys = µ.Cell([5, 7], [5.2, 8.3], µ.Formulation.small_strain)
+    mat = µ.material.MaterialLinearElastic1_2d.make(sys, "material", 210e9, .33)
+    mat.add_pixel([2, 1])
+    assert mat.size() == 1
+    assert len(sys) == 35
+    pix = list(sys)
+    assert pix[0] == [0, 0] and pix[1] == [0, 1] and pix[-1] == [4, 6]
+
+
+def test_even_grid_rejected(µ):
+    with pytest.raises(µ.ProjectionError):
+        µ.Cell([4, 5], [1., 1.], µ.Formulation.small_strain)
+
+
+def test_unassigned_pixel_raises(µ):
+    sys = µ.Cell([3, 3], [1., 1.], µ.Formulation.small_strain)
+    mat = µ.material.MaterialLinearElastic1_2d.make(sys, "m", 1., .3)
+    mat.add_pixel([0, 0])
+    with pytest.raises(RuntimeError):
+        sys.initialise()
+
+
+def test_duplicate_pixel_raises(µ):
+    sys = µ.Cell([3, 3], [1., 1.], µ.Formulation.small_strain)
+    a = µ.material.MaterialLinearElastic1_2d.make(sys, "a", 1., .3)
+    b = µ.material.MaterialLinearElastic1_2d.make(sys, "b", 2., .3)
+    for p in sys:
+        a.add_pixel(p)
+    b.add_pixel([1, 1])
+    with pytest.raises(RuntimeError):
+        sys.initialise()
+
+
+def test_solver_finite_strain_de_geus(µ):
+    sys = µ.Cell([3, 3], [3., 3.], µ.Formulation.finite_strain)
+    hard = µ.material.MaterialLinearElastic1_2d.make(sys, "hard", 210e9, .33)
+    soft = µ.material.MaterialLinearElastic1_2d.make(sys, "soft", 70e9, .33)
+    for i, pixel in enumerate(sys):
+        (hard if i < 3 else soft).add_pixel(pixel)
+    sys.initialise()
+    solver = µ.solvers.SolverCG(sys, 1e-6, 100, 0)
+    r = µ.solvers.de_geus(sys, np.array([[0, .1], [0, 0]]), solver, 1e-6, 0)
+    assert r.success and r.nb_it >= 1
+
+
+# ---- python_binding_tests.py:105-194: eigenstrain equivalence -------------
+def test_eigenstrain_solve(µ):
+    cell1 = µ.Cell([3, 3], [3., 3.], µ.Formulation.small_strain)
+    cell2 = µ.Cell([3, 3], [3., 3.], µ.Formulation.small_strain)
+    mat1 = µ.material.MaterialLinearElastic1_2d.make(cell1, "simple", 210e9, .33)
+    mat2 = µ.material.MaterialLinearElastic2_2d.make(cell2, "eigen", 210e9, .33)
+    grad = np.array([[1.1, .2], [.3, 1.5]])
+    gl_strain = -0.5 * (grad.T + grad - 2 * np.eye(2))
+    grad = -gl_strain
+    for pixel in cell1:
+        mat1.add_pixel(pixel)
+        mat2.add_pixel(pixel, gl_strain)
+    cell1.initialise()
+    cell2.initialise()
+    tol = 1e-6
+
+    def solve(cell, g):
+        solver = µ.solvers.SolverCG(cell, tol, 2, 0)
+        return µ.solvers.newton_cg(cell, g, solver, tol, tol, 0)
+
+    r1 = solve(cell1, grad)
+    r2 = solve(cell2, np.zeros_like(grad))
+    P1, P2 = r1.stress, r2.stress
+    assert np.linalg.norm(P1 - P2) / np.linalg.norm(.5 * (P1 + P2)) < tol
+    eig = cell2.get_globalised_internal_real_array("Eigenstrain")
+    assert eig.shape == (4, 9)
+    np.testing.assert_allclose(eig[:, 0], gl_strain.ravel(order="F"))
+
+
+# ---- python_material_linear_elastic{3,4}_test.py ---------------------------
+@pytest.mark.parametrize("which,res,lengths", [("MaterialLinearElastic3_2d", [5, 5], [2.5, 3.1]),
+                                               ("MaterialLinearElastic4_2d", [7, 7], [2.3, 3.9])])
+def test_le34_homogeneous(µ, which, res, lengths):
+    sys = µ.Cell(res, lengths, µ.Formulation.small_strain)
+    mat = getattr(µ.material, which).make(sys, "material")
+    Young, Poisson = 10., .3
+    for pixel in sys:
+        mat.add_pixel(pixel, Young, Poisson)
+    sys.initialise()
+    tol = 1e-6
+    Del0 = np.array([[0, 0.025], [0.025, 0]])
+    solver = µ.solvers.SolverCG(sys, tol, 100, False)
+    r = µ.solvers.newton_cg(sys, Del0, solver, tol, tol, False)
+    mu = Young / (2 * (1 + Poisson))
+    stress = 2 * mu * Del0
+    assert np.linalg.norm(r.stress.reshape(-1, 4) - stress.reshape(1, 4)) < 1e-8
+
+
+# ---- test_solver_newton_cg.cc:218-323: laminate patch test (Reuss) --------
+@pytest.mark.parametrize("driver", ["newton_cg", "de_geus"])
+def test_laminate_patch_exact(µ, driver):
+    sys = µ.Cell([3, 3], [1., 1.], µ.Formulation.small_strain)
+    contrast, Young, Poisson = 2., 2., .33
+    hard = µ.material.MaterialLinearElastic1_2d.make(sys, "hard", contrast * Young, Poisson)
+    soft = µ.material.MaterialLinearElastic1_2d.make(sys, "soft", Young, Poisson)
+    for pixel in sys:
+        (hard if pixel[0] < 1 else soft).add_pixel(pixel)
+    sys.initialise()
+    factor = 1 / contrast * 1 / 3 + 1 - 1 / 3
+    eps_soft = 1. / factor
+    eps_hard = eps_soft / contrast
+    for load, comps in [(np.array([[1., 0], [0, 0]]), [0]), (np.array([[0, 1.], [1., 0]]), [1, 2])]:
+        solver = µ.solvers.SolverCG(sys, 1e-8, 20, 0)
+        r = getattr(µ.solvers, driver)(sys, load, solver, 1e-5, 1e-10, 0)
+        F = r.grad.reshape(-1, 4)
+        for i, pixel in enumerate(sys):
+            ref = np.zeros(4)
+            ref[comps] = eps_hard if pixel[0] < 1 else eps_soft
+            assert np.linalg.norm(F[i] - ref) < 1e-12
+
+
+# ---- full newton_cg parity against the oracle ------------------------------
+def _disc(n, r):
+    x = np.arange(n)
+    return np.hypot(x[:, None] - n // 2, x[None, :] - n // 2) < r
+
+
+def test_small_case_c1_parity(µ, oracle):
+    """bin/small_case.py (config 1): 51x51, disc of radius 10, LE1, one step,
+    equil_tol = 2 as the script passes it positionally."""
+    res, lengths = [51, 51], [7., 5.]
+    mask = _disc(51, 10)
+    cell = µ.Cell(res, lengths, µ.Formulation.small_strain)
+    inc = µ.material.MaterialLinearElastic1_2d.make(cell, "incl", 10e9, .33)
+    mat = µ.material.MaterialLinearElastic1_2d.make(cell, "matrix", 70e9, .33)
+    for i, pixel in enumerate(cell):
+        (inc if mask.flat[i] else mat).add_pixel(pixel)
+    solver = µ.solvers.SolverCG(cell, 1e-8, 401, 0)
+    load = np.array([[0, 0], [0, .03]])
+    r = µ.solvers.newton_cg(cell, load, solver, 1e-5, 2, 0)
+    oc = oracle.Cell(res, lengths, oracle.SMALL_STRAIN)
+    a = oc.add_material(oracle.LE1, 10e9, .33)
+    b = oc.add_material(oracle.LE1, 70e9, .33)
+    for i in range(51 * 51):
+        oc.add_pixel(a if mask.flat[i] else b, i)
+    ro = oc.newton_cg([load], 1e-8, 401, 1e-5, 2.0)[0]
+    assert r.nb_it == ro["nb_it"] and r.nb_fev == ro["nb_fev"]
+    assert r.message == ro["message"]
+    F, P = r.grad.ravel(), r.stress.ravel()
+    assert np.linalg.norm(F - ro["grad"].ravel()) <= 1e-10 * np.linalg.norm(ro["grad"])
+    assert np.linalg.norm(P - ro["stress"].ravel()) <= 1e-10 * np.linalg.norm(ro["stress"])
+
+
+@pytest.mark.parametrize("form", ["small", "finite"])
+def test_3d_two_phase_multi_step_parity(µ, oracle, form):
+    res = [9, 11, 7]
+    g = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
+    mask = ((g[0] - 4) ** 2 + (g[1] - 5) ** 2 + (g[2] - 3) ** 2) < 9
+    F_ = µ.Formulation.small_strain if form == "small" else µ.Formulation.finite_strain
+    cell = µ.Cell(res, [1., 1.2, .8], F_)
+    h = µ.material.MaterialLinearElastic1_3d.make(cell, "hard", 100e9, .3)
+    s = µ.material.MaterialLinearElastic1_3d.make(cell, "soft", 10e9, .3)
+    idx = np.arange(mask.size)
+    h.add_pixels_by_index(idx[mask.ravel()])
+    s.add_pixels_by_index(idx[~mask.ravel()])
+    steps = [np.array([[.01, .002, 0], [.002, 0, 0], [0, 0, -.005]]) * k for k in (1, 2)]
+    solver = µ.solvers.SolverCG(cell, 1e-8, 400, 0)
+    rs = µ.solvers.newton_cg(cell, steps, solver, 1e-5, 1e-10, 0)
+    oc = oracle.Cell(res, [1., 1.2, .8], oracle.SMALL_STRAIN if form == "small" else oracle.FINITE_STRAIN)
+    a = oc.add_material(oracle.LE1, 100e9, .3)
+    b = oc.add_material(oracle.LE1, 10e9, .3)
+    for i in range(mask.size):
+        oc.add_pixel(a if mask.flat[i] else b, i)
+    ro = oc.newton_cg(steps, 1e-8, 400, 1e-5, 1e-10)
+    assert len(rs) == 2
+    for r, o in zip(rs, ro):
+        assert r.nb_it == o["nb_it"] and r.nb_fev == o["nb_fev"]
+        assert np.linalg.norm(r.grad.ravel() - o["grad"].ravel()) <= 1e-10 * np.linalg.norm(o["grad"])
+        assert np.linalg.norm(r.stress.ravel() - o["stress"].ravel()) <= 1e-10 * np.linalg.norm(o["stress"])
+
+
+def test_de_geus_equals_newton_cg(µ):
+    """test_solver_newton_cg.cc:55-99 on 5^3 (finite strain, one hard voxel)."""
+    Young, Poisson = 1.0030648180242636, 0.29930675909878679
+    out = []
+    for driver in ("de_geus", "newton_cg"):
+        sys = µ.Cell([5, 5, 5], [5., 5., 5.], µ.Formulation.finite_strain)
+        hard = µ.material.MaterialLinearElastic1_3d.make(sys, "hard", 10 * Young, Poisson)
+        soft = µ.material.MaterialLinearElastic1_3d.make(sys, "soft", Young, Poisson)
+        for i, pixel in enumerate(sys):
+            (hard if i == 0 else soft).add_pixel(pixel)
+        sys.initialise()
+        delF0 = np.zeros((3, 3))
+        delF0[0, 1] = 1.
+        solver = µ.solvers.SolverCG(sys, 1e-8, 125 * 9 * 10, 0)
+        out.append(getattr(µ.solvers, driver)(sys, [delF0], solver, 1e-5, 1e-10, 0)[0].grad)
+    assert np.abs(out[0] - out[1]).mean() <= 1e-8
+
+
+def test_convergence_error_on_tiny_maxiter(µ):
+    sys = µ.Cell([11, 11], [1., 1.], µ.Formulation.small_strain)
+    a = µ.material.MaterialLinearElastic1_2d.make(sys, "a", 100., .3)
+    b = µ.material.MaterialLinearElastic1_2d.make(sys, "b", 1., .3)
+    for i, p in enumerate(sys):
+        (a if (p[0] - 5) ** 2 + (p[1] - 5) ** 2 < 9 else b).add_pixel(p)
+    solver = µ.solvers.SolverCG(sys, 1e-12, 2, 0)
+    with pytest.raises(µ.ConvergenceError):
+        µ.solvers.newton_cg(sys, np.array([[.01, 0], [0, 0]]), solver, 1e-5, 1e-10, 0)
+
+
+def test_asymmetric_small_strain_load_raises(µ):
+    sys = µ.Cell([3, 3], [1., 1.], µ.Formulation.small_strain)
+    a = µ.material.MaterialLinearElastic1_2d.make(sys, "a", 1., .3)
+    for p in sys:
+        a.add_pixel(p)
+    solver = µ.solvers.SolverCG(sys, 1e-8, 10, 0)
+    with pytest.raises(µ.SolverError):
+        µ.solvers.newton_cg(sys, np.array([[0, 1.], [0, 0]]), solver, 1e-5, 1e-10, 0)
+
+
+def test_projection_api_vs_oracle(µ, oracle):
+    res, L = [5, 7, 9], [1., 2., 3.]
+    f = np.random.default_rng(1).standard_normal((9, int(np.prod(res))))
+    for form, kind in [(µ.Formulation.small_strain, 0), (µ.Formulation.finite_strain, 1)]:
+        proj = µ.fft.Projection(res, L, form)
+        proj.initialise()
+        out = proj.apply_projection(f)
+        ref = oracle.apply_projection(kind, res, L, np.asfortranarray(f).T.reshape(-1, 9)).T
+        assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
