@@ -61,6 +61,10 @@ extern "C" {
 #define MUGPU_LE3 3
 #define MUGPU_LE4 4
 #define MUGPU_HEP1 5
+/* HEP1 stress update with the consistent tangent dP/dF (no reference
+ * counterpart: the reference's coded HEP1 tangent, materials_toolbox.hh:
+ * 394-406, is not the derivative of P and makes CG diverge) */
+#define MUGPU_HEP1_CT 6
 
 typedef struct mugpu_fft_s *mugpu_fft;
 typedef struct mugpu_proj_s *mugpu_proj;
@@ -227,6 +231,21 @@ int mugpu_cg_solve_mixed(mugpu_cell cell, const void *d_rhs, void *d_x,
                          double tol, int maxiter, double inner_tol,
                          int64_t *iterations, int *refinements,
                          double *rel_residual);
+
+/* ---- instrumentation (no reference counterpart) ------------------------
+ * HIP-event timing of one kernel class on the library stream, used by the
+ * benchmark to measure a kernel's average duration inside a timed run. */
+#define MUGPU_K_LAST_R2C 0  /* last-axis r2c pass (with the K:p stage)      */
+#define MUGPU_K_MIDDLE 1    /* middle-axis c2c passes (3D)                  */
+#define MUGPU_K_AXIS0 2     /* first-axis pass with the projection          */
+#define MUGPU_K_LAST_C2R 3  /* last-axis c2r pass (with the p.Ap stage)     */
+#define MUGPU_K_MATERIAL 4  /* constitutive kernels                         */
+#define MUGPU_K_CG_UPDATE 5 /* x, r update with the r.r partial sums        */
+int mugpu_timing_start(int kernel_class);
+int mugpu_timing_stop(int64_t *launches, double *total_ms);
+/* streaming device copy of `bytes` (multiple of 16), the bandwidth
+ * reference for the roofline and for the HBM counter calibration */
+int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);
 
 #ifdef __cplusplus
 }

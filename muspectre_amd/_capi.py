@@ -14,7 +14,7 @@ OK, ECONVERGENCE, ESOLVER, EPROJECTION, ERUNTIME, EMEMORY, EHIP, EARG = range(8)
 FP64, FP32 = 0, 1
 FINITE_STRAIN, SMALL_STRAIN = 0, 1
 PROJ_SMALL_STRAIN, PROJ_FINITE_STRAIN_FAST, PROJ_FINITE_STRAIN = 0, 1, 2
-LE1, LE2, LE3, LE4, HEP1 = 1, 2, 3, 4, 5
+LE1, LE2, LE3, LE4, HEP1, HEP1_CT = 1, 2, 3, 4, 5, 6
 
 # name, restype, argtypes
 _vp = ctypes.c_void_p
@@ -73,7 +73,11 @@ SIGNATURES = [
     ("mugpu_cg_start", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _dp, _dp]),
     ("mugpu_cg_step", _int, [_vp, _int, _vp, _vp, _vp, _vp, _dp]),
     ("mugpu_cg_solve_mixed", _int, [_vp, _vp, _vp, _dbl, _int, _dbl, _i64p, _ip, _dp]),
+    ("mugpu_timing_start", _int, [_int]),
+    ("mugpu_timing_stop", _int, [_i64p, _dp]),
+    ("mugpu_copy", _int, [_vp, _vp, ctypes.c_size_t]),
 ]
+K_LAST_R2C, K_MIDDLE, K_AXIS0, K_LAST_C2R, K_MATERIAL, K_CG_UPDATE = range(6)
 
 _LIB = None
 

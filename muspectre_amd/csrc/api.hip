@@ -37,6 +37,33 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 hipStream_t stream() { return g_stream; }
 int max_lds_per_block() { return g_maxlds; }
 
+namespace {
+int g_timing_class = -1;
+std::vector<hipEvent_t> g_ev_start, g_ev_stop;
+int64_t g_timing_count = 0;
+constexpr int64_t kMaxTimed = 16384;
+}  // namespace
+
+void timing_begin(int kclass) {
+  if (kclass != g_timing_class || g_timing_count >= kMaxTimed) return;
+  if ((int64_t)g_ev_start.size() <= g_timing_count) {
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+    g_ev_start.push_back(a);
+    g_ev_stop.push_back(b);
+  }
+  (void)hipEventRecord(g_ev_start[g_timing_count], g_stream);
+}
+
+void timing_end(int kclass) {
+  if (kclass != g_timing_class || g_timing_count >= kMaxTimed) return;
+  if ((int64_t)g_ev_stop.size() <= g_timing_count) return;
+  (void)hipEventRecord(g_ev_stop[g_timing_count], g_stream);
+  ++g_timing_count;
+}
+
+int copy_device(void* dst, const void* src, size_t bytes);
+
 int run_tangent_product(int dim, int prec, int kind, const void* tan, const void* in, void* out,
                         int64_t npix);
 
@@ -255,6 +282,38 @@ extern "C" int mugpu_synchronize(void) {
 
 extern "C" void* mugpu_stream(void) { return (void*)g_stream; }
 
+extern "C" int mugpu_timing_start(int kernel_class) {
+  MUGPU_TRY(ensure_init());
+  g_timing_class = kernel_class;
+  g_timing_count = 0;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_timing_stop(int64_t* launches, double* total_ms) {
+  MUGPU_TRY(ensure_init());
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  double tot = 0.0;
+  for (int64_t i = 0; i < g_timing_count; ++i) {
+    float ms = 0.f;
+    MUGPU_HIP(hipEventElapsedTime(&ms, g_ev_start[i], g_ev_stop[i]));
+    tot += ms;
+  }
+  if (launches) *launches = g_timing_count;
+  if (total_ms) *total_ms = tot;
+  g_timing_class = -1;
+  g_timing_count = 0;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_copy(void* d_dst, const void* d_src, size_t bytes) {
+  MUGPU_TRY(ensure_init());
+  if (bytes % 16) {
+    set_error("mugpu_copy needs a multiple of 16 bytes");
+    return MUGPU_EARG;
+  }
+  return copy_device(d_dst, d_src, bytes);
+}
+
 // ============================== FFT engine ====================================
 extern "C" int mugpu_fft_create(int dim, const int* res, int nb_components, int precision,
                                 mugpu_fft* out) {
@@ -441,11 +500,11 @@ extern "C" int mugpu_cell_add_material(mugpu_cell c, int family, double young, d
     set_error("materials must be added before the cell is initialised");
     return MUGPU_ERUNTIME;
   }
-  if (family < MUGPU_LE1 || family > MUGPU_HEP1) {
+  if (family < MUGPU_LE1 || family > MUGPU_HEP1_CT) {
     set_error("unknown material family %d", family);
     return MUGPU_EARG;
   }
-  if (family == MUGPU_HEP1 && c->form != MUGPU_FINITE_STRAIN) {
+  if ((family == MUGPU_HEP1 || family == MUGPU_HEP1_CT) && c->form != MUGPU_FINITE_STRAIN) {
     set_error("MaterialHyperElastoPlastic1 needs the finite-strain formulation");
     return MUGPU_ERUNTIME;
   }
@@ -543,7 +602,7 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
       if (n) MUGPU_HIP(hipMemcpy(m.d_params, m.h_params.data(), sizeof(double) * n * m.ppn,
                                  hipMemcpyHostToDevice));
     }
-    if (m.family == MUGPU_HEP1) {
+    if (m.family == MUGPU_HEP1 || m.family == MUGPU_HEP1_CT) {
       // MaterialHyperElastoPlastic1::initialise (.cc:76-84): current = I,
       // then cycle; both buffers start at zero
       const size_t sz = (size_t)(n > 0 ? n : 1);
@@ -670,7 +729,7 @@ extern "C" int mugpu_cell_apply_projection(mugpu_cell c, void* d_vec) {
 extern "C" int mugpu_cell_save_history(mugpu_cell c) {
   if (!c) return MUGPU_EARG;
   for (auto& m : c->mats)
-    if (m.family == MUGPU_HEP1) m.cur ^= 1;
+    if (m.family == MUGPU_HEP1 || m.family == MUGPU_HEP1_CT) m.cur ^= 1;
   return MUGPU_OK;
 }
 
@@ -689,7 +748,7 @@ extern "C" int mugpu_cell_get_internal(mugpu_cell c, const char* name, double* h
     const double* src = nullptr;
     int k = 0;
     std::vector<double> host;
-    if (m.family == MUGPU_HEP1 && m.Fp[0]) {
+    if ((m.family == MUGPU_HEP1 || m.family == MUGPU_HEP1_CT) && m.Fp[0]) {
       if (nm == "eps_p") { src = m.ep[m.cur]; k = 1; }
       else if (nm == "F_prev") { src = m.Fp[m.cur]; k = c->nc; }
       else if (nm == "be") { src = m.be[m.cur]; k = c->nc; }

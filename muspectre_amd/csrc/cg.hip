@@ -139,7 +139,24 @@ inline unsigned grid_for(int64_t n) {
   return (unsigned)g;
 }
 
+__global__ __launch_bounds__(kRT) void k_copy(const double2* __restrict__ s,
+                                              double2* __restrict__ d, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) d[i] = s[i];
+}
+
 }  // namespace
+
+int copy_device(void* dst, const void* src, size_t bytes) {
+  const int64_t n = (int64_t)(bytes / 16);
+  int64_t g = (n + kRT - 1) / kRT;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(kRT), 0, stream(), (const double2*)src,
+                     (double2*)dst, n);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
 
 int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,
                double* d_scal, int mode) {
@@ -163,12 +180,14 @@ int finalize(const double* d_partials, int64_t count, double* d_scal, int mode) 
 int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
               int64_t n, double* d_partials) {
   const unsigned g = grid_for(n);
+  timing_begin(MUGPU_K_CG_UPDATE);
   if (prec == MUGPU_FP64)
     hipLaunchKernelGGL((k_cg_update<double>), dim3(g), dim3(kRT), 0, stream(), (double*)x,
                        (double*)r, (const double*)p, (const double*)Ap, d_scal, n, d_partials);
   else
     hipLaunchKernelGGL((k_cg_update<float>), dim3(g), dim3(kRT), 0, stream(), (float*)x,
                        (float*)r, (const float*)p, (const float*)Ap, d_scal, n, d_partials);
+  timing_end(MUGPU_K_CG_UPDATE);
   MUGPU_HIP(hipGetLastError());
   return finalize(d_partials, g, const_cast<double*>(d_scal), FIN_BETA);
 }

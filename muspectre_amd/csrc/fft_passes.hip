@@ -665,11 +665,13 @@ int launch_r2c(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
   const int nl2 = (a.rb * DIM * DIM + 1) / 2;
   const size_t lds = (size_t)nl2 * a.ls * csize<T>();
   dim3 grid((unsigned)p.nzblocks);
+  timing_begin(MUGPU_K_LAST_R2C);
   switch (op.in_mode) {
     case IN_PLAIN: hipLaunchKernelGGL((k_last_r2c<T, kNT, DIM, IN_PLAIN>), grid, dim3(kNT), lds, stream(), a); break;
     case IN_TAN: hipLaunchKernelGGL((k_last_r2c<T, kNT, DIM, IN_TAN>), grid, dim3(kNT), lds, stream(), a); break;
     default: hipLaunchKernelGGL((k_last_r2c<T, kNT, DIM, IN_TAN_PUPDATE>), grid, dim3(kNT), lds, stream(), a); break;
   }
+  timing_end(MUGPU_K_LAST_R2C);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -693,10 +695,12 @@ int launch_c2r(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
   const size_t e1 = (size_t)nl2 * a.ls, e2 = (size_t)a.rb * nh * nc;
   const size_t lds = (e1 > e2 ? e1 : e2) * csize<T>();
   dim3 grid((unsigned)p.nzblocks);
+  timing_begin(MUGPU_K_LAST_C2R);
   if (op.out_mode == OUT_DOT)
     hipLaunchKernelGGL((k_last_c2r<T, kNT, DIM, OUT_DOT>), grid, dim3(kNT), lds, stream(), a);
   else
     hipLaunchKernelGGL((k_last_c2r<T, kNT, DIM, OUT_PLAIN>), grid, dim3(kNT), lds, stream(), a);
+  timing_end(MUGPU_K_LAST_C2R);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -720,7 +724,10 @@ int launch_strided(FftPlan& p, int ax, int W, c_t<T>* spec, int sign) {
   a.tw = tw_of<T>(p, ax);
   const size_t lds = (size_t)W * a.ls * csize<T>();
   dim3 grid((unsigned)((inner + W - 1) / W), (unsigned)outer);
+  const int kcls = ax == 0 ? MUGPU_K_AXIS0 : MUGPU_K_MIDDLE;
+  timing_begin(kcls);
   hipLaunchKernelGGL((k_strided<T, kNT>), grid, dim3(kNT), lds, stream(), a);
+  timing_end(kcls);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -745,11 +752,13 @@ int launch_project(FftPlan& p, const ProjParams& pj, c_t<T>* spec) {
   a.norm = (T)pj.norm;
   const size_t lds = (size_t)a.Wk * DIM * DIM * a.ls * csize<T>();
   dim3 grid((unsigned)((a.plane + a.Wk - 1) / a.Wk));
+  timing_begin(MUGPU_K_AXIS0);
   switch (p.xNT) {
     case 256: hipLaunchKernelGGL((k_axis0_project<T, 256, DIM>), grid, dim3(256), lds, stream(), a); break;
     case 512: hipLaunchKernelGGL((k_axis0_project<T, 512, DIM>), grid, dim3(512), lds, stream(), a); break;
     default: hipLaunchKernelGGL((k_axis0_project<T, 1024, DIM>), grid, dim3(1024), lds, stream(), a); break;
   }
+  timing_end(MUGPU_K_AXIS0);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }

@@ -88,8 +88,8 @@ int64_t pixel_index(Cell& cell, const Ccoord_t<Dim>& pix) {
 // ---- per-family material classes (types for the python names) ----------------
 template <Dim_t Dim, int Fam>
 struct MaterialT : MaterialBase {
-  MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H)
-      : MaterialBase(c, n, Fam, E, nu, tau, H) {}
+  MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H, int fam = Fam)
+      : MaterialBase(c, n, fam, E, nu, tau, H) {}
 };
 
 template <Dim_t Dim, int Fam>
@@ -113,14 +113,17 @@ void bind_material(py::module& m, const std::string& stem) {
         "cell"_a, "name"_a, "Young"_a, "Poisson"_a, py::return_value_policy::reference,
         py::keep_alive<1, 0>());
   } else if (Fam == MUGPU_HEP1) {
-    // material_hyper_elasto_plastic1.cc:40-65 (not bound in the reference)
+    // material_hyper_elasto_plastic1.cc:40-65 (not bound in the reference);
+    // consistent_tangent=True keeps the stress update and replaces the
+    // coded tangent by dP/dF
     cls.def_static(
         "make",
-        [](Sys& sys, std::string n, Real E, Real nu, Real tau, Real H) -> Mat& {
-          return attach<Dim, Fam>(sys, new Mat(sys, n, E, nu, tau, H));
+        [](Sys& sys, std::string n, Real E, Real nu, Real tau, Real H, bool ct) -> Mat& {
+          return attach<Dim, Fam>(sys, new Mat(sys, n, E, nu, tau, H, ct ? MUGPU_HEP1_CT : MUGPU_HEP1));
         },
         "cell"_a, "name"_a, "Young"_a, "Poisson"_a, "tau_y0"_a, "H"_a,
-        py::return_value_policy::reference, py::keep_alive<1, 0>());
+        "consistent_tangent"_a = false, py::return_value_policy::reference,
+        py::keep_alive<1, 0>());
   } else {
     cls.def_static(
         "make",
@@ -241,6 +244,7 @@ void bind_cell(py::module& m) {
             s.evaluate_stress_tangent();
           },
           "strain"_a)
+      .def("save_history_variables", &Sys::save_history_variables)
       .def("get_subdomain_resolutions", &Sys::get_subdomain_resolutions)
       .def("get_subdomain_locations", &Sys::get_subdomain_locations)
       .def("get_domain_resolutions", &Sys::get_domain_resolutions)

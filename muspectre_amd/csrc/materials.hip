@@ -203,6 +203,132 @@ __device__ void expm(const double* T_, double* out) {
   newton_poly<D>(T_, l, exp(l[0]), exp_dd1(l[0], l[1]), d2, out);
 }
 
+// cyclic Jacobi eigen-decomposition of a symmetric 3x3 (Golub & Van Loan,
+// sec. 8.5); columns of V are the eigenvectors
+__device__ void jacobi3(const double* A, double* ev, double* V) {
+  double a[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      a[i][j] = 0.5 * (A[i + 3 * j] + A[j + 3 * i]);
+      V[i + 3 * j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+    if (off == 0.0) break;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2, k = 3 - p - q;
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double app = a[p][p], aqq = a[q][q];
+      if (sweep > 3 && fabs(app) + 100.0 * fabs(apq) == fabs(app) &&
+          fabs(aqq) + 100.0 * fabs(apq) == fabs(aqq)) {
+        a[p][q] = a[q][p] = 0.0;
+        continue;
+      }
+      const double theta = (aqq - app) / (2.0 * apq);
+      const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      const double akp = a[k][p], akq = a[k][q];
+      a[p][p] = app - t * apq;
+      a[q][q] = aqq + t * apq;
+      a[p][q] = a[q][p] = 0.0;
+      a[k][p] = a[p][k] = c * akp - s * akq;
+      a[k][q] = a[q][k] = s * akp + c * akq;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double vp = V[i + 3 * p], vq = V[i + 3 * q];
+        V[i + 3 * p] = c * vp - s * vq;
+        V[i + 3 * q] = s * vp + c * vq;
+      }
+    }
+  }
+  ev[0] = a[0][0];
+  ev[1] = a[1][1];
+  ev[2] = a[2][2];
+}
+
+// consistent tangent dP/dF of the HEP1 update (no reference counterpart; the
+// reference's coded tangent is not the derivative of P):
+//   K(i,m,j,n) = sum_k [sum_q (DL_ikjq + DL_ikqj) W_nq] Finv_mk - P_in Finv_mj
+// with W = F^-1 be*, DL = C : L, L the Daleckii-Krein derivative of ln at be*
+template <int D>
+__device__ void hep1_consistent_tangent(const double* Fi, const double* bes, const double* C,
+                                        const double* P, double* K) {
+  constexpr int NC = D * D;
+  double W[NC], ev[3], V[9];
+  matmul<D>(Fi, bes, W);
+  if (D == 3) {
+    jacobi3(bes, ev, V);
+  } else {
+    const double a = bes[0], b = 0.5 * (bes[2] + bes[1]), c = bes[3];
+    const double th = 0.5 * atan2(2 * b, a - c);
+    const double cs = cos(th), sn = sin(th);
+    V[0] = cs;
+    V[1] = sn;
+    V[2] = -sn;
+    V[3] = cs;
+    ev[0] = cs * cs * a + 2 * cs * sn * b + sn * sn * c;
+    ev[1] = sn * sn * a - 2 * cs * sn * b + cs * cs * c;
+  }
+  double g[NC];
+#pragma unroll
+  for (int m = 0; m < D; ++m)
+#pragma unroll
+    for (int n = 0; n < D; ++n) {
+      const double h = ev[n] - ev[m];
+      g[m + D * n] = h == 0.0 ? 1.0 / ev[m] : log1p(h / ev[m]) / h;
+    }
+  // L(q, c) with q = i + D j, c = k + D l; DL = C L computed column by column
+  for (int l = 0; l < D; ++l)
+    for (int k = 0; k < D; ++k) {
+      double Lc[NC];
+#pragma unroll
+      for (int j = 0; j < D; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          double s = 0;
+#pragma unroll
+          for (int m = 0; m < D; ++m)
+#pragma unroll
+            for (int n = 0; n < D; ++n)
+              s += g[m + D * n] * V[i + D * m] * V[j + D * n] * V[k + D * m] * V[l + D * n];
+          Lc[i + D * j] = s;
+        }
+      double* DLc = K + NC * (k + D * l);  // K is scratch for DL here
+#pragma unroll
+      for (int r = 0; r < NC; ++r) {
+        double s = 0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) s += C[r + NC * q] * Lc[q];
+        DLc[r] = s;
+      }
+    }
+  // E(i,k,j,n) = sum_q (DL_ikjq + DL_ikqj) W_nq, then contract with Finv
+  double Ebuf[NC * NC];
+  for (int n = 0; n < D; ++n)
+    for (int j = 0; j < D; ++j)
+      for (int k = 0; k < D; ++k)
+        for (int i = 0; i < D; ++i) {
+          double e = 0;
+#pragma unroll
+          for (int q = 0; q < D; ++q)
+            e += (K[(i + D * k) + NC * (j + D * q)] + K[(i + D * k) + NC * (q + D * j)]) * W[n + D * q];
+          Ebuf[(i + D * k) + NC * (j + D * n)] = e;
+        }
+  for (int n = 0; n < D; ++n)
+    for (int j = 0; j < D; ++j)
+      for (int m = 0; m < D; ++m)
+        for (int i = 0; i < D; ++i) {
+          double s = 0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) s += Ebuf[(i + D * k) + NC * (j + D * n)] * Fi[m + D * k];
+          K[(i + D * m) + NC * (j + D * n)] = s - P[i + D * n] * Fi[m + D * j];
+        }
+}
+
 struct MatArgs {
   const int* pix;
   int64_t n;
@@ -295,7 +421,7 @@ __global__ __launch_bounds__(256) void k_linear_elastic(MatArgs a) {
   }
 }
 
-template <int D>
+template <int D, bool CT>
 __global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
   constexpr int NC = D * D;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,10 +498,34 @@ __global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
       a.P[v * NC + i + D * j] = s;
     }
   if (!a.need_tangent) return;
-  // Kirchhoff tangent C(r,m,n,s), then M(i,m,n) = sum_r Finv(i,r) sum_s C(r,m,n,s)
   const bool plastic = phi > 0;
   const double a0 = plastic ? dg * mu / taueq : 0.0;
   const double a1 = mu / (a.H + 3 * mu);
+  if (CT) {
+    // algorithmic modulus C(r,m,n,s) as the nc x nc T4 matrix
+    double C[NC * NC];
+    for (int s = 0; s < D; ++s)
+      for (int n = 0; n < D; ++n)
+        for (int m = 0; m < D; ++m)
+          for (int r = 0; r < D; ++r) {
+            double c;
+            if (plastic) {
+              const double itrac = (r == m && n == s) ? 1.0 : 0.0;
+              const double isym = 0.5 * (((r == n && m == s) ? 1.0 : 0.0) + ((r == s && m == n) ? 1.0 : 0.0));
+              c = (a.bulk / 2. - mu / 3 + a0 * mu) * itrac + (1 - 3 * a0) * mu * isym +
+                  2 * mu * (a0 - a1) * Ns[r + D * m] * Ns[n + D * s];
+            } else {
+              c = 0.5 * iso_C<D>(lam, mu, r, m, n, s);
+            }
+            C[(r + D * m) + NC * (n + D * s)] = c;
+          }
+    double Pl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) Pl[c] = a.P[v * NC + c];
+    hep1_consistent_tangent<D>(Fi, bes, C, Pl, a.tan + v * NC * NC);
+    return;
+  }
+  // Kirchhoff tangent C(r,m,n,s), then M(i,m,n) = sum_r Finv(i,r) sum_s C(r,m,n,s)
   double csum[D * D * D];  // c(r,m,n) = sum_s C(r,m,n,s) at r + D m + D^2 n
 #pragma unroll
   for (int n = 0; n < D; ++n)
@@ -459,6 +609,10 @@ int eval_material(const MaterialDev& m, int dim, int form, const double* F, doub
   a.tan_kind = tan_kind;
   a.need_tangent = need_tangent;
   dim3 grid((unsigned)((n + 255) / 256));
+  timing_begin(MUGPU_K_MATERIAL);
+  struct End {
+    ~End() { timing_end(MUGPU_K_MATERIAL); }
+  } end_guard;
   if (m.family == MUGPU_HEP1) {
     const int cur = m.cur, old = cur ^ 1;
     a.Fp_cur = m.Fp[cur];
@@ -467,8 +621,21 @@ int eval_material(const MaterialDev& m, int dim, int form, const double* F, doub
     a.Fp_old = m.Fp[old];
     a.be_old = m.be[old];
     a.ep_old = m.ep[old];
-    if (dim == 3) hipLaunchKernelGGL((k_hep1<3>), grid, dim3(256), 0, stream(), a);
-    else hipLaunchKernelGGL((k_hep1<2>), grid, dim3(256), 0, stream(), a);
+    if (dim == 3) hipLaunchKernelGGL((k_hep1<3, false>), grid, dim3(256), 0, stream(), a);
+    else hipLaunchKernelGGL((k_hep1<2, false>), grid, dim3(256), 0, stream(), a);
+    MUGPU_HIP(hipGetLastError());
+    return MUGPU_OK;
+  }
+  if (m.family == MUGPU_HEP1_CT) {
+    const int cur = m.cur, old = cur ^ 1;
+    a.Fp_cur = m.Fp[cur];
+    a.be_cur = m.be[cur];
+    a.ep_cur = m.ep[cur];
+    a.Fp_old = m.Fp[old];
+    a.be_old = m.be[old];
+    a.ep_old = m.ep[old];
+    if (dim == 3) hipLaunchKernelGGL((k_hep1<3, true>), grid, dim3(256), 0, stream(), a);
+    else hipLaunchKernelGGL((k_hep1<2, true>), grid, dim3(256), 0, stream(), a);
     MUGPU_HIP(hipGetLastError());
     return MUGPU_OK;
   }

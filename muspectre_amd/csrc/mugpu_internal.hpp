@@ -28,6 +28,9 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 hipStream_t stream();
 int max_lds_per_block();
+// per-launch event timing of one kernel class (MUGPU_K_*), off by default
+void timing_begin(int kclass);
+void timing_end(int kclass);
 
 // ---- tangent representations of the directional-stiffness product --------
 // FULL: the reference's stored K, nc*nc per voxel, K(r,c) at r + nc*c

@@ -289,7 +289,7 @@ double *mo_cell_tangent(mo_cell *c) { return c->K; }
 int mo_cell_add_material(mo_cell *c, int family, double young, double poisson,
                          double tau_y0, double H) {
   if (c->nmat >= MO_MAXMAT) return -MO_RUNTIME_ERROR;
-  if (family == MO_HEP1 && c->form != MO_FINITE_STRAIN) {
+  if ((family == MO_HEP1 || family == MO_HEP1_CT) && c->form != MO_FINITE_STRAIN) {
     mo_set_error("HyperElastoPlastic1 needs the finite-strain formulation");
     return -MO_RUNTIME_ERROR;
   }
@@ -301,7 +301,7 @@ int mo_cell_add_material(mo_cell *c, int family, double young, double poisson,
   m->tau_y0 = tau_y0;
   m->H = H;
   const int d = c->dim;
-  if (family == MO_LE1 || family == MO_LE2 || family == MO_HEP1) {
+  if (family == MO_LE1 || family == MO_LE2 || family == MO_HEP1 || family == MO_HEP1_CT) {
     m->lambda = mo_lambda(young, poisson);
     m->mu = mo_mu(young, poisson);
     mo_hooke_C(d, m->lambda, m->mu, m->C);
@@ -374,7 +374,7 @@ int mo_cell_initialise(mo_cell *c) {
   const int nc = c->nc, d = c->dim;
   for (int mi = 0; mi < c->nmat; ++mi) {
     mo_material *m = &c->mats[mi];
-    if (m->family != MO_HEP1) continue;
+    if (m->family != MO_HEP1 && m->family != MO_HEP1_CT) continue;
     for (int b = 0; b < 2; ++b) {
       m->Fp[b] = (double *)calloc((size_t)(m->n * nc), sizeof(double));
       m->be[b] = (double *)calloc((size_t)(m->n * nc), sizeof(double));
@@ -394,7 +394,7 @@ int mo_cell_initialise(mo_cell *c) {
 
 int mo_cell_save_history(mo_cell *c) {
   for (int mi = 0; mi < c->nmat; ++mi)
-    if (c->mats[mi].family == MO_HEP1) c->mats[mi].cur ^= 1;
+    if (c->mats[mi].family == MO_HEP1 || c->mats[mi].family == MO_HEP1_CT) c->mats[mi].cur ^= 1;
   return MO_OK;
 }
 
@@ -456,6 +456,16 @@ static int eval_materials(mo_cell *c, int need_tangent) {
                                  m->ep[old][q], m->Fp[cur] + q * nc,
                                  m->be[cur] + q * nc, &m->ep[cur][q], tau, Cl);
           mo_pk1_from_kirchhoff(d, F, tau, Cl, P, K);
+          continue;
+        }
+        case MO_HEP1_CT: {
+          double tau[9], bes[9];
+          mo_hep1_eval_kirchhoff_bes(d, m->young, m->poisson, m->tau_y0, m->H, F,
+                                     m->Fp[old] + q * nc, m->be[old] + q * nc,
+                                     m->ep[old][q], m->Fp[cur] + q * nc,
+                                     m->be[cur] + q * nc, &m->ep[cur][q], tau, Cl, bes);
+          mo_pk1_from_kirchhoff(d, F, tau, Cl, P, K);
+          if (need_tangent) mo_hep1_consistent_tangent(d, F, bes, Cl, P, K);
           continue;
         }
       }
@@ -521,7 +531,7 @@ int mo_cell_get_internal(mo_cell *c, const char *name, int which, double *out,
     const int b = which == 0 ? m->cur : (m->cur ^ 1);
     const double *src = NULL;
     int k = 0;
-    if (m->family == MO_HEP1 && m->Fp[0]) {
+    if ((m->family == MO_HEP1 || m->family == MO_HEP1_CT) && m->Fp[0]) {
       if (!strcmp(name, "eps_p")) { src = m->ep[b]; k = 1; }
       else if (!strcmp(name, "F_prev")) { src = m->Fp[b]; k = c->nc; }
       else if (!strcmp(name, "be")) { src = m->be[b]; k = c->nc; }

@@ -9,6 +9,12 @@
 #include <string.h>
 
 void mo_inverse(int d, const double *A, double *inv);
+void mo_hep1_eval_kirchhoff_bes(int d, double young, double poisson,
+                                double tau_y0, double H, const double *F,
+                                const double *F_prev_old, const double *be_old,
+                                double eps_old, double *F_prev_cur,
+                                double *be_cur, double *eps_cur, double *tau,
+                                double *C, double *bes_out);
 void mo_logm(int d, const double *T, double *out);
 void mo_expm(int d, const double *T, double *out);
 
@@ -137,6 +143,16 @@ void mo_hep1_eval_kirchhoff(int d, double young, double poisson,
                             double eps_old, double *F_prev_cur,
                             double *be_cur, double *eps_cur, double *tau,
                             double *C) {
+  mo_hep1_eval_kirchhoff_bes(d, young, poisson, tau_y0, H, F, F_prev_old, be_old, eps_old,
+                             F_prev_cur, be_cur, eps_cur, tau, C, NULL);
+}
+
+void mo_hep1_eval_kirchhoff_bes(int d, double young, double poisson,
+                                double tau_y0, double H, const double *F,
+                                const double *F_prev_old, const double *be_old,
+                                double eps_old, double *F_prev_cur,
+                                double *be_cur, double *eps_cur, double *tau,
+                                double *C, double *bes_out) {
   const double lam = mo_lambda(young, poisson), mu = mo_mu(young, poisson);
   const double Kb = mo_bulk(young, poisson);
   double Fpi[9], f[9], tmp[9], bes[9], lnb[9], ts[9], td[9], Ns[9];
@@ -161,6 +177,7 @@ void mo_hep1_eval_kirchhoff(int d, double young, double poisson,
       for (int k = 0; k < d; ++k) s += tmp[i + d * k] * f[j + d * k];
       bes[i + d * j] = s;
     }
+  if (bes_out) memcpy(bes_out, bes, sizeof(double) * d * d);
   mo_logm(d, bes, lnb);
   /* tau* = 0.5 * Hooke(ln be*) */
   mo_hooke_stress(d, lam, mu, lnb, ts);
@@ -202,4 +219,81 @@ void mo_hep1_eval_kirchhoff(int d, double young, double poisson,
     mo_hooke_C(d, lam, mu, C);
     for (int k = 0; k < d * d * d * d; ++k) C[k] *= 0.5;
   }
+}
+
+void mo_jacobi_sym3(const double *A, double *ev, double *V);
+
+/*
+ * Consistent tangent K(i,m,j,n) = dP_im/dF_jn of the HEP1 stress update
+ * (no reference counterpart: the reference's coded tangent,
+ * materials_toolbox.hh:394-406, is not the derivative of P and makes CG
+ * diverge).  With be* = F A F^T (A fixed by the history), W = F^-1 be*,
+ * dbe* = dF W + W^T dF^T, d(ln be*) = L : dbe* (Daleckii-Krein,
+ * L_ijkl = sum_mn g_mn v_mi v_nj v_mk v_nl, g_mn = (ln l_n - ln l_m)/(l_n - l_m),
+ * g_mm = 1/l_m), dtau = C : d(ln be*) with C the algorithmic modulus of
+ * the return mapping (the reference's C, material_hyper_elasto_plastic1.hh:
+ * 284-309), and P = tau F^-T:
+ *   K(i,m,j,n) = sum_k [sum_q (DL_ikjq + DL_ikqj) W_nq] Finv_mk - P_in Finv_mj,
+ *   DL = C : L.
+ * C is the dim^2 x dim^2 T4 matrix returned by mo_hep1_eval_kirchhoff.
+ */
+void mo_hep1_consistent_tangent(int d, const double *F, const double *bes,
+                                const double *C, const double *P, double *K) {
+  const int nc = d * d;
+  double Fi[9], W[9], ev[3], V[9], L[81], DL[81];
+  mo_inverse(d, F, Fi);
+  for (int j = 0; j < d; ++j)
+    for (int i = 0; i < d; ++i) {
+      double s = 0;
+      for (int k = 0; k < d; ++k) s += Fi[i + d * k] * bes[k + d * j];
+      W[i + d * j] = s;
+    }
+  if (d == 3) {
+    mo_jacobi_sym3(bes, ev, V);
+  } else {
+    /* 2x2: rotation angle in closed form */
+    const double a = bes[0], b = 0.5 * (bes[2] + bes[1]), c = bes[3];
+    const double th = 0.5 * atan2(2 * b, a - c);
+    const double cs = cos(th), sn = sin(th);
+    V[0] = cs; V[1] = sn; V[2] = -sn; V[3] = cs;
+    ev[0] = cs * cs * a + 2 * cs * sn * b + sn * sn * c;
+    ev[1] = sn * sn * a - 2 * cs * sn * b + cs * cs * c;
+  }
+  double g[9];
+  for (int m = 0; m < d; ++m)
+    for (int n = 0; n < d; ++n) {
+      const double h = ev[n] - ev[m];
+      g[m + d * n] = h == 0.0 ? 1.0 / ev[m] : log1p(h / ev[m]) / h;
+    }
+  for (int l = 0; l < d; ++l)
+    for (int k = 0; k < d; ++k)
+      for (int j = 0; j < d; ++j)
+        for (int i = 0; i < d; ++i) {
+          double s = 0;
+          for (int m = 0; m < d; ++m)
+            for (int n = 0; n < d; ++n)
+              s += g[m + d * n] * V[i + d * m] * V[j + d * n] * V[k + d * m] * V[l + d * n];
+          L[(i + d * j) + nc * (k + d * l)] = s;
+        }
+  /* DL = C : L as nc x nc matrices */
+  for (int c = 0; c < nc; ++c)
+    for (int r = 0; r < nc; ++r) {
+      double s = 0;
+      for (int q = 0; q < nc; ++q) s += C[r + nc * q] * L[q + nc * c];
+      DL[r + nc * c] = s;
+    }
+  for (int n = 0; n < d; ++n)
+    for (int j = 0; j < d; ++j)
+      for (int m = 0; m < d; ++m)
+        for (int i = 0; i < d; ++i) {
+          double s = 0;
+          for (int k = 0; k < d; ++k) {
+            double e = 0;
+            for (int q = 0; q < d; ++q)
+              e += (DL[(i + d * k) + nc * (j + d * q)] + DL[(i + d * k) + nc * (q + d * j)]) *
+                   W[n + d * q];
+            s += e * Fi[m + d * k];
+          }
+          K[(i + d * m) + nc * (j + d * n)] = s - P[i + d * n] * Fi[m + d * j];
+        }
 }

@@ -264,3 +264,49 @@ void mo_inverse(int d, const double *A, double *inv) {
   inv[2 + 3 * 2] = (a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0)) * id;
 #undef a
 }
+
+/* cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix (Golub & Van
+ * Loan, "Matrix Computations", sec. 8.5): A = V diag(ev) V^T, columns of V
+ * are the eigenvectors.  Orthonormal to rounding also for repeated
+ * eigenvalues, which the consistent tangent's divided differences need. */
+void mo_jacobi_sym3(const double *A, double *ev, double *V) {
+  double a[3][3];
+  for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i) {
+      a[i][j] = 0.5 * (A[i + 3 * j] + A[j + 3 * i]);
+      V[i + 3 * j] = (i == j) ? 1.0 : 0.0;
+    }
+  static const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+    if (off == 0.0) break;
+    for (int r = 0; r < 3; ++r) {
+      const int p = P[r], q = Q[r];
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double app = a[p][p], aqq = a[q][q];
+      /* skip rotations below the rounding of the diagonal */
+      if (sweep > 3 && fabs(app) + 100.0 * fabs(apq) == fabs(app) &&
+          fabs(aqq) + 100.0 * fabs(apq) == fabs(aqq)) {
+        a[p][q] = a[q][p] = 0.0;
+        continue;
+      }
+      const double theta = (aqq - app) / (2.0 * apq);
+      const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      const int k = 3 - p - q;
+      const double akp = a[k][p], akq = a[k][q];
+      a[p][p] = app - t * apq;
+      a[q][q] = aqq + t * apq;
+      a[p][q] = a[q][p] = 0.0;
+      a[k][p] = a[p][k] = c * akp - s * akq;
+      a[k][q] = a[q][k] = s * akp + c * akq;
+      for (int i = 0; i < 3; ++i) {
+        const double vp = V[i + 3 * p], vq = V[i + 3 * q];
+        V[i + 3 * p] = c * vp - s * vq;
+        V[i + 3 * q] = s * vp + c * vq;
+      }
+    }
+  }
+  for (int i = 0; i < 3; ++i) ev[i] = a[i][i];
+}

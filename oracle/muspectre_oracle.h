@@ -48,8 +48,10 @@ enum {
   MO_PROJ_FINITE_STRAIN = 2       /* projection_finite_strain.cc (full Ghat) */
 };
 
+/* MO_HEP1_CT: the HEP1 stress update with the consistent tangent dP/dF
+ * (no reference counterpart; see mo_hep1_consistent_tangent) */
 enum {
-  MO_LE1 = 1, MO_LE2 = 2, MO_LE3 = 3, MO_LE4 = 4, MO_HEP1 = 5
+  MO_LE1 = 1, MO_LE2 = 2, MO_LE3 = 3, MO_LE4 = 4, MO_HEP1 = 5, MO_HEP1_CT = 6
 };
 
 /* threads used by the OpenMP loops (1 = the reference's single-core path) */
@@ -101,6 +103,18 @@ void mo_hep1_eval_kirchhoff(int dim, double young, double poisson,
                             double eps_old, double *F_prev_cur,
                             double *be_cur, double *eps_cur, double *tau,
                             double *C);
+/* same, also returning the trial elastic left Cauchy-Green tensor be* */
+void mo_hep1_eval_kirchhoff_bes(int dim, double young, double poisson,
+                                double tau_y0, double H, const double *F,
+                                const double *F_prev_old, const double *be_old,
+                                double eps_old, double *F_prev_cur,
+                                double *be_cur, double *eps_cur, double *tau,
+                                double *C, double *bes);
+/* consistent tangent dP/dF of the HEP1 update (no reference counterpart);
+ * C is the algorithmic modulus from mo_hep1_eval_kirchhoff, P = tau F^-T */
+void mo_hep1_consistent_tangent(int dim, const double *F, const double *bes,
+                                const double *C, const double *P, double *K);
+void mo_jacobi_sym3(const double *A, double *evals, double *evecs);
 /* PK1 conversions (src/materials/materials_toolbox.hh:313-410) */
 void mo_pk1_from_pk2(int dim, const double *F, const double *S,
                      const double *C, double *P, double *K);

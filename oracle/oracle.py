@@ -20,7 +20,7 @@ _LIB = None
 
 FINITE_STRAIN, SMALL_STRAIN = 0, 1
 PROJ_SMALL, PROJ_FINITE_FAST, PROJ_FINITE = 0, 1, 2
-LE1, LE2, LE3, LE4, HEP1 = 1, 2, 3, 4, 5
+LE1, LE2, LE3, LE4, HEP1, HEP1_CT = 1, 2, 3, 4, 5, 6
 
 ERRORS = {1: "ConvergenceError", 2: "SolverError", 3: "ProjectionError",
           4: "RuntimeError", 5: "MemoryError"}
@@ -87,6 +87,11 @@ def lib():
         L.mo_hep1_eval_kirchhoff.argtypes = [
             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
             ctypes.c_double, dp, dp, dp, ctypes.c_double, dp, dp, dp, dp, dp]
+        L.mo_hep1_eval_kirchhoff_bes.argtypes = [
+            ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+            ctypes.c_double, dp, dp, dp, ctypes.c_double, dp, dp, dp, dp, dp, dp]
+        L.mo_hep1_consistent_tangent.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
+        L.mo_jacobi_sym3.argtypes = [dp, dp, dp]
         L.mo_pk1_from_pk2.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.mo_pk1_from_kirchhoff.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.mo_set_threads.argtypes = [ctypes.c_int]
@@ -221,6 +226,40 @@ def hep1_eval(dim, young, poisson, tau_y0, H, F, F_prev_old, be_old, eps_old,
                                  _dp(Fp), _dp(be), float(eps_old), _dp(Fpc),
                                  _dp(bec), _dp(epc), _dp(tau), _dp(C))
     return tau, C.reshape(dim * dim, dim * dim, order="F"), Fpc, bec, epc[0]
+
+
+def hep1_stress_and_tangent(dim, young, poisson, tau_y0, H, F, F_prev_old,
+                            be_old, eps_old, consistent=True):
+    """PK1 stress P and tangent K (dim^2 x dim^2 T4 matrix) of HEP1 at F with
+    the given history; consistent=False gives the tangent as the reference
+    codes it (materials_toolbox.hh:374-410)."""
+    f = lambda a: np.asfortranarray(a, dtype=np.float64)
+    F, Fp, be = f(F), f(F_prev_old), f(be_old)
+    Fpc = np.zeros((dim, dim), order="F")
+    bec = np.zeros((dim, dim), order="F")
+    epc = np.zeros(1)
+    tau = np.zeros((dim, dim), order="F")
+    C = np.zeros(dim ** 4)
+    bes = np.zeros((dim, dim), order="F")
+    lib().mo_hep1_eval_kirchhoff_bes(dim, young, poisson, tau_y0, H, _dp(F),
+                                     _dp(Fp), _dp(be), float(eps_old), _dp(Fpc),
+                                     _dp(bec), _dp(epc), _dp(tau), _dp(C),
+                                     _dp(bes))
+    P = np.zeros((dim, dim), order="F")
+    K = np.zeros(dim ** 4)
+    lib().mo_pk1_from_kirchhoff(dim, _dp(F), _dp(tau), _dp(C), _dp(P), _dp(K))
+    if consistent:
+        lib().mo_hep1_consistent_tangent(dim, _dp(F), _dp(bes), _dp(C), _dp(P),
+                                         _dp(K))
+    return P, K.reshape(dim * dim, dim * dim, order="F")
+
+
+def jacobi_sym3(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    ev = np.zeros(3)
+    V = np.zeros((3, 3), order="F")
+    lib().mo_jacobi_sym3(_dp(A), _dp(ev), _dp(V))
+    return ev, V
 
 
 class Result(ctypes.Structure):
