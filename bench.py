@@ -1,0 +1,392 @@
+#!/usr/bin/env python3
+"""Benchmark of the Newton-CG solve on one MI355X.
+
+Metric: voxel-CG-iters/s = N_voxels x (CG iterations counted by SolverCG)
+/ wall time of the newton_cg call (cell construction and initialise are
+outside the timed region).  One step = one complete newton_cg solve of the
+workload from a freshly initialised cell.
+
+Workloads (synthetic inputs generated from fixed seeds):
+  c3  256^3 finite strain, two HyperElastoPlastic1 phases from a Gaussian
+      random field (correlation length 16 voxels) thresholded at its median;
+      soft E=70 GPa nu=.33 tau_y0=200 MPa H=1 GPa, hard E=210 GPa
+      tau_y0=800 MPa; first load step of the uniaxial ramp, F_zz = 1.005;
+      consistent tangent (the coded HEP1 tangent makes CG diverge);
+      one Newton iteration (return mapping, residual projection and the CG
+      solve to tol 1e-8)
+  c2  128^3 small strain, 40 periodic spheres (r ~ U[4,10]) of E=100 GPa in
+      E=10 GPa, nu=.3, six unit load cases, CG tol 1e-8
+  c5  512^3 small strain LE4, E lognormal (median 50 GPa, sigma_log .8),
+      nu ~ U[.15,.45], shear eps_xy = .01, CG tol 1e-8
+  c1  51x51 small strain, disc of radius 10 (bin/small_case.py)
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c5", "c1"])
+    ap.add_argument("--size", type=int, default=0, help="override the grid size")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--kernel", default="auto",
+                    choices=["auto", "r2c", "middle", "axis0", "c2r", "material", "update"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+# ---------------------------------------------------------------- workloads
+def grf_mask(n, ell, seed):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((n, n, n))
+    k = np.fft.fftfreq(n)
+    f = np.fft.rfftn(w)
+    kz = np.fft.rfftfreq(n)
+    K2 = k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2
+    f *= np.exp(-0.5 * (2 * np.pi) ** 2 * K2 * ell ** 2)
+    g = np.fft.irfftn(f, s=(n, n, n), axes=(0, 1, 2))
+    return g > np.median(g)
+
+
+def spheres_mask(n, count, seed):
+    rng = np.random.default_rng(seed)
+    x = np.arange(n)
+    mask = np.zeros((n, n, n), dtype=bool)
+    for _ in range(count):
+        c = rng.uniform(0, n, 3)
+        r = rng.uniform(4, 10)
+        d = [np.minimum(np.abs(x - ci), n - np.abs(x - ci)) for ci in c]
+        mask |= (d[0][:, None, None] ** 2 + d[1][None, :, None] ** 2 +
+                 d[2][None, None, :] ** 2) < r * r
+    return mask
+
+
+class Workload:
+    def __init__(self, args):
+        self.name = args.workload
+        self.prec = args.precision
+        w = self.name
+        if w == "c3":
+            self.n = args.size or 256
+            self.res = [self.n] * 3
+            self.mask = grf_mask(self.n, 16.0 * self.n / 256.0, 2026)
+            self.form = "finite"
+            self.loads = [[np.diag([0.0, 0.0, 0.005])]]
+            self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 4000
+            self.equil_tol = 1e-10
+            # one Newton iteration of the first load step: on fine grids the
+            # later Newton iterations meet indefinite tangents (finite-strain
+            # geometric stress with the near-deviatoric-free return-mapping
+            # modulus of a large plastic step) on which CG diverges
+            self.newton_maxiter = 1
+            self.B_alg = 1728  # 9*(6*8+9*8) + 648: full 81-double tangent
+        elif w == "c2":
+            self.n = args.size or 128
+            self.res = [self.n] * 3
+            self.mask = spheres_mask(self.n, 40, 2026)
+            self.form = "small"
+            eye = np.eye(3)
+            cases = []
+            for (i, j) in [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]:
+                e = np.zeros((3, 3))
+                e[i, j] = e[j, i] = 1.0
+                cases.append([e])
+            self.loads = cases
+            self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 4000
+            self.equil_tol = 1e-10
+            self.B_alg = 1096  # 9*(6*8+9*8) + 16: (lambda, mu) per voxel
+        elif w == "c5":
+            self.n = args.size or 512
+            self.res = [self.n] * 3
+            rng = np.random.default_rng(2026)
+            N = self.n ** 3
+            self.E = np.exp(np.log(50e9) + 0.8 * rng.standard_normal(N))
+            self.nu = rng.uniform(0.15, 0.45, N)
+            self.form = "small"
+            e = np.zeros((3, 3))
+            e[0, 1] = e[1, 0] = 0.01
+            self.loads = [[e]]
+            self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 4000
+            self.equil_tol = 1e-10
+            self.B_alg = 1096
+        else:
+            self.n = args.size or 51
+            self.res = [self.n, self.n]
+            x = np.arange(self.n)
+            c = self.n // 2
+            self.mask = np.hypot(x[:, None] - c, x[None, :] - c) < 10 * self.n / 51
+            self.form = "small"
+            self.loads = [[np.array([[0, 0], [0, .03]])]]
+            self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 401
+            self.equil_tol = 2.0
+            self.B_alg = 4 * (6 * 8 + 9 * 8) + 16
+        self.N = int(np.prod(self.res))
+        if not hasattr(self, "newton_maxiter"):
+            self.newton_maxiter = 0
+        self.converged = None
+
+    def make_cell(self, mu):
+        dim = len(self.res)
+        form = mu.Formulation.finite_strain if self.form == "finite" else mu.Formulation.small_strain
+        lengths = [1.0] * dim if self.name != "c1" else [7.0, 5.0]
+        cell = mu.Cell(self.res, lengths, form, precision=self.prec,
+                       allow_even=any(r % 2 == 0 for r in self.res))
+        idx = np.arange(self.N, dtype=np.int64)
+        m3 = mu.material
+        if self.name == "c3":
+            soft = m3.MaterialHyperElastoPlastic1_3d.make(cell, "soft", 70e9, .33, 200e6, 1e9,
+                                                          consistent_tangent=True)
+            hard = m3.MaterialHyperElastoPlastic1_3d.make(cell, "hard", 210e9, .33, 800e6, 1e9,
+                                                          consistent_tangent=True)
+            hard.add_pixels_by_index(idx[self.mask.ravel()])
+            soft.add_pixels_by_index(idx[~self.mask.ravel()])
+        elif self.name == "c2":
+            inc = m3.MaterialLinearElastic1_3d.make(cell, "inclusion", 100e9, .3)
+            mat = m3.MaterialLinearElastic1_3d.make(cell, "matrix", 10e9, .3)
+            inc.add_pixels_by_index(idx[self.mask.ravel()])
+            mat.add_pixels_by_index(idx[~self.mask.ravel()])
+        elif self.name == "c5":
+            m = m3.MaterialLinearElastic4_3d.make(cell, "lognormal")
+            m.add_pixels_by_index(idx, np.stack([self.E, self.nu], axis=1))
+        else:
+            inc = m3.MaterialLinearElastic1_2d.make(cell, "inclusion", 10e9, .33)
+            mat = m3.MaterialLinearElastic1_2d.make(cell, "matrix", 70e9, .33)
+            inc.add_pixels_by_index(idx[self.mask.ravel()])
+            mat.add_pixels_by_index(idx[~self.mask.ravel()])
+        cell.initialise()
+        return cell
+
+    def solve(self, mu, cell):
+        """All load cases of one step; returns (newton iterations, CG iterations)."""
+        its = cg = 0
+        for steps in self.loads:
+            solver = mu.solvers.SolverCG(cell, self.cg_tol, self.maxiter, False)
+            res = mu.solvers.newton_cg(cell, list(steps), solver, self.newton_tol,
+                                       self.equil_tol, 0, newton_maxiter=self.newton_maxiter)
+            self.converged = all(r.success for r in res)
+            its += sum(r.nb_it for r in res)
+            cg += res[-1].nb_fev
+        return its, cg
+
+
+# -------------------------------------------------------- kernel accounting
+K_IDS = {"r2c": 0, "middle": 1, "axis0": 2, "c2r": 3, "material": 4, "update": 5}
+K_NAMES = {0: "k_last_r2c", 1: "k_strided (middle axis)", 2: "k_axis0_project",
+           3: "k_last_c2r", 4: "material", 5: "k_cg_update"}
+
+
+def kernel_bytes(wl, kclass, prec):
+    """Algorithmic HBM bytes of one launch: each input read once and each
+    output written once.  For the spectrum passes this is a read and a write
+    of the Hermitian half spectrum, Kpts x nc complex values."""
+    dim = len(wl.res)
+    nc = dim * dim
+    s = 8 if prec == "fp64" else 4
+    herm = list(wl.res[:-1]) + [wl.res[-1] // 2 + 1]
+    kpts = int(np.prod(herm))
+    if kclass in (1, 2):
+        return 2 * kpts * nc * 2 * s
+    if kclass == 5:
+        return 6 * wl.N * nc * s
+    return None
+
+
+# ------------------------------------------------------------- CPU baseline
+def cpu_baseline(wl, budget_s):
+    from oracle import oracle as O
+    O.set_threads(1)
+    if wl.name in ("c3", "c2"):
+        n = 48 if wl.name == "c3" else 64
+        res = [n] * 3
+    elif wl.name == "c5":
+        n = 64
+        res = [n] * 3
+    else:
+        n = wl.n
+        res = [n, n]
+    N = int(np.prod(res))
+    dim = len(res)
+    if wl.name == "c3":
+        mask = grf_mask(n, 16.0 * n / 256.0, 2026)
+        c = O.Cell(res, [1.0] * 3, O.FINITE_STRAIN, allow_even=True)
+        a = c.add_material(O.HEP1_CT, 70e9, .33, 200e6, 1e9)
+        b = c.add_material(O.HEP1_CT, 210e9, .33, 800e6, 1e9)
+        for i in range(N):
+            c.add_pixel(b if mask.flat[i] else a, i)
+        c.initialise()
+        c.F[:] = np.eye(3).ravel(order="F") + np.diag([0, 0, .005]).ravel(order="F")
+    else:
+        mask = spheres_mask(n, 40, 2026) if dim == 3 else wl.mask
+        c = O.Cell(res, [1.0] * dim, O.SMALL_STRAIN, allow_even=n % 2 == 0)
+        a = c.add_material(O.LE1, 100e9, .3)
+        b = c.add_material(O.LE1, 10e9, .3)
+        for i in range(N):
+            c.add_pixel(a if mask.flat[i] else b, i)
+        c.initialise()
+        e = np.zeros((dim, dim))
+        e[0, 1] = e[1, 0] = 0.01
+        c.F[:] = e.ravel(order="F")
+    c.evaluate_stress_tangent()
+    rhs = -c.apply_projection(c.P.copy())
+    c.cg_fixed_iterations(rhs, 1)  # warm-up
+    iters = 0
+    t0 = time.perf_counter()
+    while True:
+        c.cg_fixed_iterations(rhs, 1)
+        iters += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or iters >= 200:
+            break
+    return {"value": N * iters / el, "unit": "voxel-CG-iters/s", "cores": 1, "kind": "port",
+            "sample": "{} {} voxels, {} CG iterations of the oracle (FP64, stored 9x9 "
+                      "tangent and projection operator as in the reference), {:.1f} s"
+                      .format(wl.name, "x".join(map(str, res)), iters, el)}
+
+
+def copy_bandwidth(capi, nbytes=4 << 30, reps=5):
+    lib = capi.load()
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(lib.mugpu_alloc(nbytes, ctypes.byref(a)))
+    capi.check(lib.mugpu_alloc(nbytes, ctypes.byref(b)))
+    capi.check(lib.mugpu_copy(b, a, nbytes))
+    capi.check(lib.mugpu_synchronize())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        capi.check(lib.mugpu_copy(b, a, nbytes))
+    capi.check(lib.mugpu_synchronize())
+    dt = (time.perf_counter() - t0) / reps
+    lib.mugpu_free(a)
+    lib.mugpu_free(b)
+    return 2 * nbytes / dt / 1e9
+
+
+def traffic_from_profiles(wl, kname):
+    """HBM bytes per launch measured with rocprofv3 PMC counters, if a
+    summary for this workload and kernel is committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == wl.name + ":" + str(wl.n) and parts[1] == kname:
+            try:
+                return float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: every box here has one GPU and this build "
+              "has no multi-GPU path (--gpus {} requested)".format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+    import muspectre_amd as mu
+    from muspectre_amd import _capi
+    lib = _capi.load()
+    _capi.check(lib.mugpu_init(0))
+    wl = Workload(args)
+    print("bench: workload {} {} {} ({} voxels), precision {}".format(
+        wl.name, "x".join(map(str, wl.res)), wl.form, wl.N, wl.prec), file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        cell = wl.make_cell(mu)
+        its, cg = wl.solve(mu, cell)
+        print("bench: warmup {}: {} Newton, {} CG iterations".format(w, its, cg),
+              file=sys.stderr, flush=True)
+        del cell
+    kchoice = args.kernel
+    if kchoice == "auto":
+        kchoice = "middle" if len(wl.res) == 3 else "axis0"
+    kclass = K_IDS[kchoice]
+    total = 0.0
+    setup = 0.0
+    tot_its = tot_cg = 0
+    launches_all = 0
+    kms_all = 0.0
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        cell = wl.make_cell(mu)
+        _capi.check(lib.mugpu_synchronize())
+        setup += time.perf_counter() - t0
+        _capi.check(lib.mugpu_timing_start(kclass))
+        _capi.check(lib.mugpu_synchronize())
+        t0 = time.perf_counter()
+        its, cg = wl.solve(mu, cell)
+        _capi.check(lib.mugpu_synchronize())
+        dt = time.perf_counter() - t0
+        launches = ctypes.c_int64()
+        kms = ctypes.c_double()
+        _capi.check(lib.mugpu_timing_stop(ctypes.byref(launches), ctypes.byref(kms)))
+        launches_all += launches.value
+        kms_all += kms.value
+        total += dt
+        tot_its += its
+        tot_cg += cg
+        print("bench: step {}: {:.3f} s, {} Newton, {} CG iterations".format(s, dt, its, cg),
+              file=sys.stderr, flush=True)
+        del cell
+    value = wl.N * tot_cg / total
+    kb = kernel_bytes(wl, kclass, wl.prec)
+    roof = None
+    if launches_all and kb:
+        avg = kms_all / launches_all / 1e3
+        ach = kb / avg / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": traffic_from_profiles(wl, K_NAMES[kclass]),
+                "kernel": K_NAMES[kclass], "launches": launches_all,
+                "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": kb}
+    copy_gbs = copy_bandwidth(_capi)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, args.cpu_seconds)
+    out = {
+        "metric": "voxel-CG-iters/s",
+        "value": round(value, 1),
+        "unit": "voxel-CG-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * total / args.steps, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if wl.prec == "fp64" else "f32-operator/f64-refined",
+        "data": "synthetic",
+        "config": {"workload": wl.name, "grid": "x".join(map(str, wl.res)), "formulation": wl.form,
+                   "newton_iterations": tot_its, "cg_iterations": tot_cg,
+                   "load_cases": len(wl.loads), "precision": wl.prec,
+                   "newton_tol": wl.newton_tol, "cg_tol": wl.cg_tol,
+                   "newton_maxiter": wl.newton_maxiter or None, "converged": wl.converged,
+                   "setup_s_per_step": round(setup / args.steps, 2)},
+        "roofline": roof,
+        "whole_iteration_roofline": {
+            "bytes_per_voxel_iter": wl.B_alg,
+            "achieved": round(wl.B_alg * value / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(wl.B_alg * value / 1e9 / HBM_PEAK_GBS, 4)},
+        "copy_kernel_GBs": round(copy_gbs, 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -391,14 +391,14 @@ void bind_common_helpers(py::module& m) {
 }
 
 py::object solve_driver(bool geus, Cell& cell, py::object steps_obj, SolverBase& solver,
-                        Real newton_tol, Real equil_tol, Dim_t verbose) {
+                        Real newton_tol, Real equil_tol, Dim_t verbose, Uint newton_maxiter) {
   const bool is_list = py::isinstance<py::list>(steps_obj) || py::isinstance<py::tuple>(steps_obj);
   LoadSteps_t steps = is_list ? steps_from(steps_obj.cast<py::sequence>())
                               : LoadSteps_t{matrix_from(steps_obj.cast<rarr>())};
   std::vector<OptimizeResult> res;
   {
-    res = geus ? de_geus(cell, steps, solver, newton_tol, equil_tol, verbose)
-               : newton_cg(cell, steps, solver, newton_tol, equil_tol, verbose);
+    res = geus ? de_geus(cell, steps, solver, newton_tol, equil_tol, verbose, newton_maxiter)
+               : newton_cg(cell, steps, solver, newton_tol, equil_tol, verbose, newton_maxiter);
   }
   auto conv = [](const OptimizeResult& r) {
     OptimizeResult c = r;
@@ -513,28 +513,32 @@ PYBIND11_MODULE(_muSpectre, mod) {
   // list overload: kwarg equilibrium_tol (:95-104)
   solvers.def(
       "newton_cg",
-      [](Cell& c, rarr g, SolverBase& so, Real nt, Real eqt, Dim_t verb) {
-        return solve_driver(false, c, g, so, nt, eqt, verb);
+      [](Cell& c, rarr g, SolverBase& so, Real nt, Real eqt, Dim_t verb, Uint nmax) {
+        return solve_driver(false, c, g, so, nt, eqt, verb, nmax);
       },
-      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equil_tol"_a, "verbose"_a = 0);
+      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equil_tol"_a, "verbose"_a = 0,
+      py::kw_only(), "newton_maxiter"_a = 0);
   solvers.def(
       "newton_cg",
-      [](Cell& c, py::list g, SolverBase& so, Real nt, Real eqt, Dim_t verb) {
-        return solve_driver(false, c, g, so, nt, eqt, verb);
+      [](Cell& c, py::list g, SolverBase& so, Real nt, Real eqt, Dim_t verb, Uint nmax) {
+        return solve_driver(false, c, g, so, nt, eqt, verb, nmax);
       },
-      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0);
+      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0,
+      py::kw_only(), "newton_maxiter"_a = 0);
   solvers.def(
       "de_geus",
-      [](Cell& c, rarr g, SolverBase& so, Real nt, Real eqt, Dim_t verb) {
-        return solve_driver(true, c, g, so, nt, eqt, verb);
+      [](Cell& c, rarr g, SolverBase& so, Real nt, Real eqt, Dim_t verb, Uint nmax) {
+        return solve_driver(true, c, g, so, nt, eqt, verb, nmax);
       },
-      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0);
+      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0,
+      py::kw_only(), "newton_maxiter"_a = 0);
   solvers.def(
       "de_geus",
-      [](Cell& c, py::list g, SolverBase& so, Real nt, Real eqt, Dim_t verb) {
-        return solve_driver(true, c, g, so, nt, eqt, verb);
+      [](Cell& c, py::list g, SolverBase& so, Real nt, Real eqt, Dim_t verb, Uint nmax) {
+        return solve_driver(true, c, g, so, nt, eqt, verb, nmax);
       },
-      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0);
+      "cell"_a, "ΔF₀"_a, "solver"_a, "newton_tol"_a, "equilibrium_tol"_a, "verbose"_a = 0,
+      py::kw_only(), "newton_maxiter"_a = 0);
 
   // ---- fft engines and projections (bind_py_fftengine.cc, bind_py_projections.cc) ----
   auto fft = mod.def_submodule("fft");

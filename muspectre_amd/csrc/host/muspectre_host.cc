@@ -539,7 +539,8 @@ void print_header(const std::string& kind, Cell& cell, SolverBase& solver, Real 
 // shared body of newton_cg (src/solver/solvers.cc:44-217) and de_geus (:220-408)
 std::vector<OptimizeResult> drive(bool geus, Cell& cell, const LoadSteps_t& load_steps,
                                   SolverBase& solver, Real newton_tol, Real equil_tol,
-                                  Dim_t verbose) {
+                                  Dim_t verbose, Uint newton_maxiter) {
+  const Uint newton_bound = newton_maxiter ? newton_maxiter : solver.get_maxiter();
   const int64_t ndof = cell.get_nb_dof();
   DeviceVector incrF(ndof), rhs(ndof), DeltaF;
   if (geus) DeltaF.resize(ndof);
@@ -563,7 +564,7 @@ std::vector<OptimizeResult> drive(bool geus, Cell& cell, const LoadSteps_t& load
     if (!geus) cell.add_uniform_strain(delta);
     ConvergenceState cs{2 * newton_tol, 1.0, 2 * equil_tol, newton_tol, equil_tol};
     Uint newt_iter = 0;
-    for (; newt_iter < solver.get_maxiter() && !cs.has_converged; ++newt_iter) {
+    for (; newt_iter < newton_bound && !cs.has_converged; ++newt_iter) {
       cell.evaluate_stress_tangent();
       if (geus && newt_iter == 0) {
         // DeltaF = uniform macro increment; rhs = -A DeltaF (solvers.cc:353-363)
@@ -619,14 +620,14 @@ std::vector<OptimizeResult> drive(bool geus, Cell& cell, const LoadSteps_t& load
 
 std::vector<OptimizeResult> newton_cg(Cell& cell, const LoadSteps_t& load_steps,
                                       SolverBase& solver, Real newton_tol, Real equil_tol,
-                                      Dim_t verbose) {
-  return drive(false, cell, load_steps, solver, newton_tol, equil_tol, verbose);
+                                      Dim_t verbose, Uint newton_maxiter) {
+  return drive(false, cell, load_steps, solver, newton_tol, equil_tol, verbose, newton_maxiter);
 }
 
 std::vector<OptimizeResult> de_geus(Cell& cell, const LoadSteps_t& load_steps,
                                     SolverBase& solver, Real newton_tol, Real equil_tol,
-                                    Dim_t verbose) {
-  return drive(true, cell, load_steps, solver, newton_tol, equil_tol, verbose);
+                                    Dim_t verbose, Uint newton_maxiter) {
+  return drive(true, cell, load_steps, solver, newton_tol, equil_tol, verbose, newton_maxiter);
 }
 
 }  // namespace muSpectre

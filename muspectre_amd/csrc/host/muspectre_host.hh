@@ -362,12 +362,15 @@ class SolverCG : public SolverBase {
 };
 
 // src/solver/solvers.hh:54-74
+// newton_maxiter: optional bound on the Newton iterations per load step
+// (no reference counterpart; 0 keeps the reference's bound, the solver's
+// maxiter, src/solver/solvers.cc:165-168)
 std::vector<OptimizeResult> newton_cg(Cell& cell, const LoadSteps_t& load_steps,
                                       SolverBase& solver, Real newton_tol, Real equil_tol,
-                                      Dim_t verbose = 0);
+                                      Dim_t verbose = 0, Uint newton_maxiter = 0);
 std::vector<OptimizeResult> de_geus(Cell& cell, const LoadSteps_t& load_steps,
                                     SolverBase& solver, Real newton_tol, Real equil_tol,
-                                    Dim_t verbose = 0);
+                                    Dim_t verbose = 0, Uint newton_maxiter = 0);
 
 // src/solver/solver_common.cc:39-43
 bool check_symmetry(const Matrix& eps, Real rel_tol = 1e-8);
