@@ -1,0 +1,202 @@
+"""CPU checks that pin the oracle's constitutive laws and tensor functions.
+
+Pins: the reference's golden values for HEP1 (two states) and expm;
+numpy.linalg.eigh for logm/expm on separated and repeated spectra; the
+explicit Curnier forms of the LE1 finite-strain P and K used by the
+reference (tests/test_goodies.hh:94-134); closed-form Hooke stresses
+(test_material_linear_elastic3/4.cc); the elastic-modulus identities
+(test_materials_toolbox.cc:211-268); central finite differences for every
+tangent that is a derivative.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "reference_vectors.json")))
+
+
+def _hep1_params():
+    g = GOLDEN["hep1"]
+    K, mu = g["K"], g["mu"]
+    E = 9 * K * mu / (3 * K + mu)
+    nu = (3 * K - 2 * mu) / (2 * (3 * K + mu))
+    return E, nu, g["tau_y0"], g["H"]
+
+
+def test_hep1_golden_two_states(oracle):
+    E, nu, ty, H = _hep1_params()
+    I = np.eye(3)
+    g = GOLDEN["hep1"]
+    F = np.eye(3)
+    F[0, 1] = g["state1"]["F01"]
+    tau, C, Fpc, bec, epc = oracle.hep1_eval(3, E, nu, ty, H, F, I, I, 0.0, 0.0)
+    s = g["state1"]
+    assert np.linalg.norm(tau - np.array(s["tau"])) < 1e-8
+    assert np.linalg.norm(bec - np.array(s["be"])) < 1e-8
+    assert abs(epc - s["eps_p"]) < 1e-8
+    assert np.linalg.norm(C - np.array(s["C4"])) < 1e-8
+    # second state: old values from the first evaluation, current buffer
+    # holds the value of two cycles ago (statefield.hh:318-326): zero
+    F2 = np.eye(3)
+    F2[0, 1] = g["state2"]["F01"]
+    tau2, C2, _, bec2, epc2 = oracle.hep1_eval(3, E, nu, ty, H, F2, Fpc, bec, epc, 0.0)
+    s = g["state2"]
+    assert np.linalg.norm(tau2 - np.array(s["tau"])) < 1e-8
+    assert np.linalg.norm(bec2 - np.array(s["be"])) < 1e-8
+    assert abs(epc2 - s["eps_p"]) < 1e-8
+    assert np.linalg.norm(C2 - np.array(s["C4"])) < 1e-8
+
+
+def test_hep1_eps_p_accumulates_on_current_buffer(oracle):
+    """material_hyper_elasto_plastic1.hh:262: eps_p.current() += dgamma."""
+    E, nu, ty, H = _hep1_params()
+    F = np.eye(3)
+    F[0, 1] = 0.2
+    _, _, _, _, e1 = oracle.hep1_eval(3, E, nu, ty, H, F, np.eye(3), np.eye(3), 0.0, 0.0)
+    _, _, _, _, e2 = oracle.hep1_eval(3, E, nu, ty, H, F, np.eye(3), np.eye(3), 0.0, e1)
+    assert e1 > 0 and abs(e2 - 2 * e1) < 1e-15
+
+
+def test_expm_golden(oracle):
+    g = GOLDEN["expm"]
+    A = np.array(g["input"])
+    for fn in (oracle.expm, oracle.expm_eig):
+        assert np.linalg.norm(fn(A) - np.array(g["expected"])) < 1e-8
+
+
+def _spd(rng, spread):
+    Q = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+    return Q @ np.diag(1 + spread * rng.random(3)) @ Q.T
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_logm_expm_vs_eigh(oracle, seed):
+    rng = np.random.default_rng(seed)
+    A = _spd(rng, 0.5)
+    w, V = np.linalg.eigh(A)
+    assert np.abs(oracle.logm(A) - V @ np.diag(np.log(w)) @ V.T).max() < 1e-14
+    S = 0.3 * (A - np.eye(3))
+    w, V = np.linalg.eigh(S)
+    assert np.abs(oracle.expm(S) - V @ np.diag(np.exp(w)) @ V.T).max() < 1e-14
+    # the reference's literal Sylvester form agrees on separated spectra
+    assert np.abs(oracle.logm_sylvester(A) - oracle.logm(A)).max() < 1e-13
+
+
+@pytest.mark.parametrize("diag", [[1, 1, 1.010025], [0.99, 1.02, 1.02], [1.1, 1.1, 1.1]])
+def test_logm_expm_repeated_eigenvalues(oracle, diag):
+    rng = np.random.default_rng(1)
+    Q = np.linalg.qr(rng.standard_normal((3, 3)))[0]
+    A = Q @ np.diag(diag) @ Q.T
+    A = 0.5 * (A + A.T)
+    ref = Q @ np.diag(np.log(diag)) @ Q.T
+    assert np.abs(oracle.logm(A) - ref).max() < 1e-13
+    assert np.abs(oracle.expm(oracle.logm(A)) - A).max() < 1e-13
+
+
+def test_logm_2d(oracle):
+    A = np.array([[1.2, 0.1], [0.1, 0.9]])
+    w, V = np.linalg.eigh(A)
+    assert np.abs(oracle.logm(A) - V @ np.diag(np.log(w)) @ V.T).max() < 1e-14
+
+
+def test_jacobi_eigensystem(oracle):
+    rng = np.random.default_rng(7)
+    for spread in (0.5, 1e-9, 0.0):
+        A = _spd(rng, spread)
+        ev, V = oracle.jacobi_sym3(A)
+        assert np.abs(V.T @ V - np.eye(3)).max() < 1e-14
+        assert np.abs(V @ np.diag(ev) @ V.T - A).max() < 1e-14
+
+
+def _lame(E, nu):
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def test_le1_finite_strain_vs_curnier(oracle):
+    """P and K of LE1 in finite strain against the explicit forms of Curnier
+    (tests/test_goodies.hh:94-134, test_material_linear_elastic1.cc:134-234)."""
+    E, nu = 210e9, .33
+    lam, mu = _lame(E, nu)
+    F = np.eye(3) + 0.05 * np.random.default_rng(2).standard_normal((3, 3))
+    P, K = oracle.le1_eval(3, 0, E, nu, F)
+    I = np.eye(3)
+    Fr = (F * F).sum()
+    Pc = lam / 2 * (Fr - 3) * F + mu * F @ (F.T @ F - I)
+    assert np.abs(P - Pc).max() < 1e-12 * np.abs(Pc).max()
+    FtF, FFt = F.T @ F, F @ F.T
+    Kc = np.zeros((9, 9))
+    for i in range(3):
+        for j in range(3):
+            for m in range(3):
+                for n in range(3):
+                    Kc[i + 3 * m, j + 3 * n] = (lam * ((Fr - 3) / 2 * I[i, j] * I[m, n] + F[i, m] * F[j, n]) +
+                                                mu * (I[i, j] * FtF[m, n] + FFt[i, j] * I[m, n] -
+                                                      I[i, j] * I[m, n] + F[i, n] * F[j, m]))
+    assert np.abs(K - Kc).max() < 1e-12 * np.abs(Kc).max()
+
+
+def test_le1_small_strain_hooke(oracle):
+    E, nu = 10., .3
+    lam, mu = _lame(E, nu)
+    eps = np.array([[0.01, 0.002, 0], [0.002, -0.004, 0.001], [0, 0.001, 0.003]])
+    P, K = oracle.le1_eval(3, 1, E, nu, eps)
+    assert np.abs(P - (lam * np.trace(eps) * np.eye(3) + 2 * mu * eps)).max() < 1e-15
+    # the tangent is the Hooke tensor: contracting it with eps gives the stress
+    assert np.abs(K @ eps.ravel(order="F") - P.ravel(order="F")).max() < 1e-15
+
+
+def test_moduli_identities():
+    """materials_toolbox.hh:488-525 conversions round-trip
+    (test_materials_toolbox.cc:211-268)."""
+    E, nu = 70e9, .33
+    lam, mu = _lame(E, nu)
+    K = E / (3 * (1 - 2 * nu))
+    assert abs(9 * K * mu / (3 * K + mu) - E) < 1e-4
+    assert abs((3 * K - 2 * mu) / (2 * (3 * K + mu)) - nu) < 1e-14
+    assert abs(mu * (3 * lam + 2 * mu) / (lam + mu) - E) < 1e-4
+
+
+def _fd_tangent(fun, F, h=1e-7):
+    d = F.shape[0]
+    K = np.zeros((d * d, d * d))
+    for j in range(d):
+        for n in range(d):
+            dF = np.zeros_like(F)
+            dF[j, n] = h
+            K[:, j + d * n] = ((fun(F + dF) - fun(F - dF)) / (2 * h)).ravel(order="F")
+    return K
+
+
+def test_pk2_to_pk1_tangent_is_derivative(oracle):
+    """materials_toolbox.hh:313-350: K = dP/dF for P = F S(E(F))."""
+    E, nu = 210e9, .33
+    F = np.eye(3) + 0.03 * np.random.default_rng(4).standard_normal((3, 3))
+    _, K = oracle.le1_eval(3, 0, E, nu, F)
+    Kfd = _fd_tangent(lambda G: oracle.le1_eval(3, 0, E, nu, G)[0], F)
+    assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_hep1_consistent_tangent_is_derivative(oracle, seed):
+    rng = np.random.default_rng(seed)
+    E, nu, ty, H = 70e9, .33, 200e6, 1e9
+    Fp = np.eye(3) + 0.01 * rng.standard_normal((3, 3))
+    be = np.eye(3) + 0.002 * rng.standard_normal((3, 3))
+    be = 0.5 * (be + be.T)
+    F = Fp + (0.003 if seed % 2 else 0.0005) * rng.standard_normal((3, 3))
+    P, K = oracle.hep1_stress_and_tangent(3, E, nu, ty, H, F, Fp, be, 0.001)
+    Kfd = _fd_tangent(lambda G: oracle.hep1_stress_and_tangent(3, E, nu, ty, H, G, Fp, be, 0.001,
+                                                               False)[0], F)
+    assert np.abs(K - Kfd).max() < 1e-7 * np.abs(Kfd).max()
+
+
+def test_hep1_coded_tangent_is_j_independent(oracle):
+    """materials_toolbox.hh:394-406 as coded: K(i,m,j,n) does not depend on j."""
+    E, nu, ty, H = 70e9, .33, 200e6, 1e9
+    F = np.eye(3) + 0.004 * np.random.default_rng(0).standard_normal((3, 3))
+    _, K = oracle.hep1_stress_and_tangent(3, E, nu, ty, H, F, np.eye(3), np.eye(3), 0.0, False)
+    K4 = K.reshape(3, 3, 3, 3, order="F")  # (i, m, j, n)
+    assert np.abs(K4 - K4[:, :, :1, :]).max() == 0.0
