@@ -138,6 +138,11 @@ class Workload:
             self.equil_tol = 2.0
             self.B_alg = 4 * (6 * 8 + 9 * 8) + 16
         self.N = int(np.prod(self.res))
+        if self.prec == "fp32":
+            # FP32 operator fields and CG vectors, FP32 tangent copy
+            nc = len(self.res) ** 2
+            T = {"c3": 81 * 4, "c2": 8, "c5": 8, "c1": 8}[self.name]
+            self.B_alg = nc * (6 * 4 + 9 * 4) + T
         if not hasattr(self, "newton_maxiter"):
             self.newton_maxiter = 0
         self.converged = None
@@ -181,6 +186,7 @@ class Workload:
             res = mu.solvers.newton_cg(cell, list(steps), solver, self.newton_tol,
                                        self.equil_tol, 0, newton_maxiter=self.newton_maxiter)
             self.converged = all(r.success for r in res)
+            self.last = res
             its += sum(r.nb_it for r in res)
             cg += res[-1].nb_fev
         return its, cg
@@ -293,6 +299,46 @@ def traffic_from_profiles(wl, kname):
     return None
 
 
+def homogenised_stress_at(mu, wl, grad):
+    """<P> evaluated at a strain field on a fresh FP64 cell (initial history),
+    so that two solutions are compared on the same footing: newton_cg reports
+    the stress of the last evaluated strain, which precedes the last update."""
+    nc = len(wl.res) ** 2
+    prec = wl.prec
+    wl.prec = "fp64"
+    cell = wl.make_cell(mu)
+    wl.prec = prec
+    cell.evaluate_stress_tangent(np.asfortranarray(np.asarray(grad).reshape(-1, nc).T))
+    out = np.asarray(cell.get_stress()).reshape(nc, -1).mean(1)
+    del cell
+    return out
+
+
+def mixed_precision_error(mu, wl):
+    """Untimed: the workload again in plain FP64 on the GPU (itself held to
+    1e-10 of the FP64 oracle by the parity tests), then the relative error of
+    the mixed-precision result in homogenised stress and in the strain field
+    (F - I for finite strain)."""
+    nc = len(wl.res) ** 2
+    g32 = [np.asarray(r.grad).reshape(-1, nc) for r in wl.last]
+    wl.prec = "fp64"
+    cell = wl.make_cell(mu)
+    wl.solve(mu, cell)
+    del cell
+    g64 = [np.asarray(r.grad).reshape(-1, nc) for r in wl.last]
+    wl.prec = "fp32"
+    eye = np.eye(len(wl.res)).ravel() if wl.form == "finite" else 0.0
+    es = ef = 0.0
+    for a, b in zip(g32, g64):
+        s32 = homogenised_stress_at(mu, wl, a)
+        s64 = homogenised_stress_at(mu, wl, b)
+        es = max(es, float(np.linalg.norm(s32 - s64) / np.linalg.norm(s64)))
+        ef = max(ef, float(np.linalg.norm(a - b) / np.linalg.norm(b - eye)))
+    return {"homogenised_stress_rel": es, "strain_field_rel": ef,
+            "against": "plain FP64 GPU solve of the same workload",
+            "strain_measure": "F - I" if wl.form == "finite" else "eps"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -354,6 +400,12 @@ def main():
                 "traffic": traffic_from_profiles(wl, K_NAMES[kclass]),
                 "kernel": K_NAMES[kclass], "launches": launches_all,
                 "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": kb}
+    fp32_error = None
+    if wl.prec == "fp32":
+        fp32_error = mixed_precision_error(mu, wl)
+        print("bench: fp32 error vs fp64: stress {:.3e}, strain {:.3e}".format(
+            fp32_error["homogenised_stress_rel"], fp32_error["strain_field_rel"]),
+            file=sys.stderr, flush=True)
     copy_gbs = copy_bandwidth(_capi)
     cpu = None
     if not args.no_cpu_baseline:
@@ -383,6 +435,7 @@ def main():
             "achieved": round(wl.B_alg * value / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(wl.B_alg * value / 1e9 / HBM_PEAK_GBS, 4)},
         "copy_kernel_GBs": round(copy_gbs, 1),
+        "fp32_error": fp32_error,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -400,13 +400,9 @@ py::object solve_driver(bool geus, Cell& cell, py::object steps_obj, SolverBase&
     res = geus ? de_geus(cell, steps, solver, newton_tol, equil_tol, verbose, newton_maxiter)
                : newton_cg(cell, steps, solver, newton_tol, equil_tol, verbose, newton_maxiter);
   }
-  auto conv = [](const OptimizeResult& r) {
-    OptimizeResult c = r;
-    return c;
-  };
-  if (!is_list) return py::cast(conv(res.front()));
+  if (!is_list) return py::cast(std::move(res.front()));
   py::list out;
-  for (auto& r : res) out.append(py::cast(conv(r)));
+  for (auto& r : res) out.append(py::cast(std::move(r)));
   return std::move(out);
 }
 
@@ -485,11 +481,11 @@ PYBIND11_MODULE(_muSpectre, mod) {
   py::class_<OptimizeResult>(mod, "OptimizeResult")
       .def(py::init<>())
       .def_property("grad",
-                    [](const OptimizeResult& r) { return field_to(r.grad, (int64_t)r.grad.size(), 1); },
-                    [](OptimizeResult& r, farr a) { r.grad.assign(a.data(), a.data() + a.size()); })
+                    [](const OptimizeResult& r) { return field_to(r.grad.host(), r.grad.size(), 1); },
+                    [](OptimizeResult& r, farr a) { r.grad.assign(a.data(), (int64_t)a.size()); })
       .def_property("stress",
-                    [](const OptimizeResult& r) { return field_to(r.stress, (int64_t)r.stress.size(), 1); },
-                    [](OptimizeResult& r, farr a) { r.stress.assign(a.data(), a.data() + a.size()); })
+                    [](const OptimizeResult& r) { return field_to(r.stress.host(), r.stress.size(), 1); },
+                    [](OptimizeResult& r, farr a) { r.stress.assign(a.data(), (int64_t)a.size()); })
       .def_readwrite("success", &OptimizeResult::success)
       .def_readwrite("status", &OptimizeResult::status)
       .def_readwrite("message", &OptimizeResult::message)

@@ -56,6 +56,27 @@ std::vector<Real> DeviceVector::to_host() const {
 }
 void DeviceVector::set_zero() { check(mugpu_memset_zero(ptr_, sizeof(Real) * n_)); }
 
+// ---- ResultField ---------------------------------------------------------------
+void ResultField::capture(const void* d_src, int64_t n) {
+  dev_ = std::make_shared<DeviceVector>(n);
+  check(mugpu_scale_copy(1.0, d_src, dev_->data(), n));
+  host_.clear();
+  host_valid_ = false;
+}
+const std::vector<Real>& ResultField::host() const {
+  if (!host_valid_) {
+    host_.assign((size_t)(dev_ ? dev_->size() : 0), 0.0);
+    if (dev_) dev_->download(host_.data());
+    host_valid_ = true;
+  }
+  return host_;
+}
+void ResultField::assign(const Real* h, int64_t n) {
+  dev_.reset();
+  host_.assign(h, h + n);
+  host_valid_ = true;
+}
+
 // ---- FFT engine ------------------------------------------------------------------
 template <Dim_t Dim>
 FFTEngineBase<Dim>::FFTEngineBase(Ccoord res, Dim_t nbc)
@@ -603,8 +624,8 @@ std::vector<OptimizeResult> drive(bool geus, Cell& cell, const LoadSteps_t& load
     }
     previous = macro;
     OptimizeResult r;
-    r.grad = cell.get_strain();
-    r.stress = cell.get_stress();
+    r.grad.capture(cell.strain_data(), ndof);
+    r.stress.capture(cell.stress_data(), ndof);
     r.success = cs.test();
     r.status = Int(r.success);
     r.message = cs.message;

@@ -91,10 +91,27 @@ class DeviceVector {
   int64_t n_{0};
 };
 
+// A field of a result record: a device copy taken when the record is made
+// (one HBM-speed copy), downloaded to the host on first access and kept.
+class ResultField {
+ public:
+  ResultField() = default;
+  // copies n values from device memory
+  void capture(const void* d_src, int64_t n);
+  int64_t size() const { return host_valid_ ? (int64_t)host_.size() : (dev_ ? dev_->size() : 0); }
+  const std::vector<Real>& host() const;
+  void assign(const Real* h, int64_t n);
+
+ private:
+  std::shared_ptr<DeviceVector> dev_;
+  mutable std::vector<Real> host_;
+  mutable bool host_valid_{false};
+};
+
 // src/solver/solver_common.hh:50-67
 struct OptimizeResult {
-  std::vector<Real> grad;    // nb_dof values (AoS)
-  std::vector<Real> stress;  // nb_dof values (AoS)
+  ResultField grad;    // nb_dof values (AoS)
+  ResultField stress;  // nb_dof values (AoS)
   bool success{false};
   Int status{0};
   std::string message;
