@@ -576,10 +576,20 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
       return MUGPU_ERUNTIME;
     }
   // tangent representation
-  bool all_hep = !c->mats.empty();
-  for (auto& m : c->mats) all_hep = all_hep && m.family == MUGPU_HEP1;
-  c->tan_kind = c->form == MUGPU_SMALL_STRAIN ? TAN_ISO : (all_hep ? TAN_HEP : TAN_FULL);
-  const size_t tn = (size_t)c->N * tangent_stride(c->tan_kind, c->dim);
+  // small strain: every law here is isotropic Hooke per voxel -> (lambda, mu);
+  // finite strain: the coded HEP1 tangent alone -> its dim^3 form; laws whose
+  // tangent is major-symmetric (LE1-4 pushed forward from PK2, the
+  // consistent HEP1 tangent; 3D) -> upper triangle; otherwise the full matrix
+  bool all_hep = !c->mats.empty(), all_sym = c->dim == 3 && !c->mats.empty();
+  for (auto& m : c->mats) {
+    all_hep = all_hep && m.family == MUGPU_HEP1;
+    all_sym = all_sym && m.family != MUGPU_HEP1;
+  }
+  c->tan_kind = c->form == MUGPU_SMALL_STRAIN ? TAN_ISO
+                : all_hep                     ? TAN_HEP
+                : all_sym                     ? TAN_SYM
+                                              : TAN_FULL;
+  const size_t tn = (size_t)tangent_values(c->tan_kind, c->dim, c->N);
   if (hipMalloc((void**)&c->tan64, tn * sizeof(double)) != hipSuccess) {
     set_error("device allocation of the tangent failed");
     return MUGPU_EMEMORY;
@@ -656,7 +666,7 @@ static int evaluate(mugpu_cell c, int need_tangent) {
   for (auto& m : c->mats)
     MUGPU_TRY(eval_material(m, c->dim, c->form, c->F, c->P, c->tan_kind, c->tan64, need_tangent));
   if (need_tangent && c->prec == MUGPU_FP32) {
-    const int64_t tn = c->N * tangent_stride(c->tan_kind, c->dim);
+    const int64_t tn = tangent_values(c->tan_kind, c->dim, c->N);
     MUGPU_TRY(convert(MUGPU_FP64, c->tan64, MUGPU_FP32, c->tan32, tn, 1.0));
   }
   return MUGPU_OK;
@@ -671,13 +681,17 @@ extern "C" int mugpu_cell_get_tangent(mugpu_cell c, double* h_K) {
     return MUGPU_ERUNTIME;
   }
   const int d = c->dim, nc = c->nc, st = tangent_stride(c->tan_kind, d);
-  std::vector<double> t((size_t)c->N * st);
+  std::vector<double> t((size_t)tangent_values(c->tan_kind, d, c->N));
   MUGPU_TRY(mugpu_download(t.data(), c->tan64, t.size() * sizeof(double)));
   for (int64_t v = 0; v < c->N; ++v) {
     double* K = h_K + v * nc * nc;
-    const double* s = t.data() + v * st;
+    auto s = [&](int q) { return t[(size_t)tan_off(v, q, st)]; };
     if (c->tan_kind == TAN_FULL) {
-      std::memcpy(K, s, sizeof(double) * nc * nc);
+      for (int q = 0; q < nc * nc; ++q) K[q] = s(q);
+    } else if (c->tan_kind == TAN_SYM) {
+      for (int col = 0; col < nc; ++col)
+        for (int row = 0; row < nc; ++row)
+          K[row + nc * col] = row <= col ? s(sym_index(row, col, nc)) : s(sym_index(col, row, nc));
     } else if (c->tan_kind == TAN_ISO) {
       for (int l = 0; l < d; ++l)
         for (int k = 0; k < d; ++k)
@@ -685,13 +699,13 @@ extern "C" int mugpu_cell_get_tangent(mugpu_cell c, double* h_K) {
             for (int i = 0; i < d; ++i) {
               const double itrac = (i == j && k == l) ? 1.0 : 0.0;
               const double isym = 0.5 * (((i == k && j == l) ? 1.0 : 0.0) + ((i == l && j == k) ? 1.0 : 0.0));
-              K[(i + d * j) + nc * (k + d * l)] = s[0] * itrac + 2 * s[1] * isym;
+              K[(i + d * j) + nc * (k + d * l)] = s(0) * itrac + 2 * s(1) * isym;
             }
     } else {
       for (int n = 0; n < d; ++n)
         for (int j = 0; j < d; ++j)
           for (int m = 0; m < d; ++m)
-            for (int i = 0; i < d; ++i) K[(i + d * m) + nc * (j + d * n)] = s[i + d * m + d * d * n];
+            for (int i = 0; i < d; ++i) K[(i + d * m) + nc * (j + d * n)] = s(i + d * m + d * d * n);
     }
   }
   return MUGPU_OK;

@@ -154,6 +154,47 @@ struct Butterfly<T, 3> {
   }
 };
 
+// radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2,
+//   X[k1 + 4 k2] = sum_n2 W4^(n2 k2) W16^(n2 k1) sum_n1 W4^(n1 k1) x[4 n1 + n2]
+template <typename T>
+struct Butterfly<T, 16> {
+  __device__ static void run(c_t<T>* v, int sign, const c_t<T>*, int) {
+    const T c1 = T(0.92387953251128675612818318939678828682);  // cos(pi/8)
+    const T s1 = T(0.38268343236508977172845998403039886676);  // sin(pi/8)
+    const T h = T(0.70710678118654752440084436210484903928);
+    // cos and sin of 2 pi m / 16 for m = 0..9
+    const T cs[10] = {T(1), c1, h, s1, T(0), -s1, -h, -c1, T(-1), -c1};
+    const T sn[10] = {T(0), s1, h, c1, T(1), c1, h, s1, T(0), -s1};
+    c_t<T> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      c_t<T> x[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Butterfly<T, 4>::run(x, sign, nullptr, 0);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) {
+        const int m = n2 * k1;
+        if (m == 0) {
+          a[n2][k1] = x[k1];
+        } else if (m == 4) {
+          a[n2][k1] = mul_mi(x[k1], sign);
+        } else {
+          c_t<T> w;
+          w.x = cs[m];
+          w.y = sign < 0 ? -sn[m] : sn[m];
+          a[n2][k1] = cmul(x[k1], w);
+        }
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      c_t<T> y[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Butterfly<T, 4>::run(y, sign, nullptr, 0);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = y[k2];
+    }
+  }
+};
+
 // small odd radix by direct sums with roots W_R^m = tw[m * n/R]
 template <typename T, int R>
 struct Butterfly {

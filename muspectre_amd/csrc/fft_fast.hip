@@ -1,0 +1,538 @@
+// FFT passes for axis lengths known at compile time (powers of two from 16
+// to 512: the grids of the large workloads).  Same stages and fusions as the
+// runtime-length passes in fft_passes.hip, laid out for CDNA:
+//
+//  * lines across lanes: a workgroup owns L lines (columns of a tile) and
+//    P = n/16 threads per line; thread (l, p) holds 16 complex values of
+//    line l in registers.  Consecutive lanes work on consecutive lines, so
+//    every LDS access of a pass is unit stride across lanes (element m of
+//    line l at s[m * LS + l], LS = L + 1), and on the strided axes every HBM
+//    access is a contiguous run of L values;
+//  * register-resident Stockham passes of radix 16/8/4 with every index a
+//    compile-time shift or mask (no integer division), one LDS exchange
+//    between passes;
+//  * on the strided axes the first pass reads HBM straight into registers
+//    and the last pass writes HBM straight from registers.
+//
+// Stockham pass of radix R after a span NS (product of the earlier radices),
+// butterfly j < n/R, k = j mod NS:
+//   v_t = x[j + t n/R] W_{NS R}^{t k},  v = DFT_R(v),  y[(j/NS) NS R + k + t NS] = v_t
+#include "mugpu_internal.hpp"
+#include "op_stages.hpp"
+
+namespace mugpu {
+
+bool fast_size(int n) {
+  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+}
+
+namespace {
+
+constexpr int kE = 16;  // complex values per thread
+
+__host__ __device__ constexpr int fast_npass(int n) { return n == 16 ? 1 : n == 512 ? 3 : 2; }
+__host__ __device__ constexpr int fast_radix(int n, int i) {
+  return n == 16 ? 16 : n == 32 ? (i == 0 ? 8 : 4) : n == 64 ? 8 : n == 128 ? (i == 0 ? 16 : 8)
+                                                                         : n == 256 ? 16 : 8;
+}
+
+template <int N, int R>
+__device__ __forceinline__ int in_idx(int p, int b, int t) {
+  return p + (N / kE) * b + t * (N / R);
+}
+template <int N, int R, int NS>
+__device__ __forceinline__ int out_idx(int p, int b, int t) {
+  const int j = p + (N / kE) * b;
+  return (j / NS) * NS * R + (j & (NS - 1)) + t * NS;
+}
+
+// twiddles and butterflies of pass (R, NS) on this thread's 16 values
+template <typename T, int N, int R, int NS, int SG>
+__device__ __forceinline__ void bfly_pass(c_t<T>* v, int p, const c_t<T>* __restrict__ tw) {
+  constexpr int BPT = kE / R;
+  constexpr int TWS = N / (NS * R);
+#pragma unroll
+  for (int b = 0; b < BPT; ++b) {
+    if (NS > 1) {
+      const int k = (p + (N / kE) * b) & (NS - 1);
+#pragma unroll
+      for (int t = 1; t < R; ++t) {
+        c_t<T> w = tw[t * k * TWS];
+        if (SG > 0) w.y = -w.y;
+        v[b * R + t] = cmul(v[b * R + t], w);
+      }
+    }
+    Butterfly<T, R>::run(v + b * R, SG, nullptr, 0);
+  }
+}
+
+// Runs pass I (inputs in v) and the passes after it; between passes the
+// values go through the workgroup's LDS lines (at(m) is element m of this
+// thread's line).  On return v holds the outputs of the last pass, to be
+// stored at out_idx<N, R_last, N / R_last>.
+template <typename T, int N, int SG, int I, int NS>
+struct Passes {
+  static constexpr int R = fast_radix(N, I);
+  template <class At>
+  __device__ static void run(c_t<T>* v, int p, const c_t<T>* __restrict__ tw, At at) {
+    bfly_pass<T, N, R, NS, SG>(v, p, tw);
+    if constexpr (I + 1 < fast_npass(N)) {
+      constexpr int R2 = fast_radix(N, I + 1);
+#pragma unroll
+      for (int b = 0; b < kE / R; ++b)
+#pragma unroll
+        for (int t = 0; t < R; ++t) at(out_idx<N, R, NS>(p, b, t)) = v[b * R + t];
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < kE / R2; ++b)
+#pragma unroll
+        for (int t = 0; t < R2; ++t) v[b * R2 + t] = at(in_idx<N, R2>(p, b, t));
+      __syncthreads();
+      Passes<T, N, SG, I + 1, NS * R>::run(v, p, tw, at);
+    }
+  }
+};
+
+template <int N>
+struct Last {
+  static constexpr int R = fast_radix(N, fast_npass(N) - 1);
+  static constexpr int NS = N / R;
+};
+
+// load the inputs of the first pass: v[b R0 + t] = ld(in index)
+template <typename T, int N, class Ld>
+__device__ __forceinline__ void load_first(c_t<T>* v, int p, Ld ld) {
+  constexpr int R0 = fast_radix(N, 0);
+#pragma unroll
+  for (int b = 0; b < kE / R0; ++b)
+#pragma unroll
+    for (int t = 0; t < R0; ++t) v[b * R0 + t] = ld(in_idx<N, R0>(p, b, t));
+}
+// store the outputs of the last pass: st(out index, value)
+template <typename T, int N, class St>
+__device__ __forceinline__ void store_last(const c_t<T>* v, int p, St st) {
+  constexpr int R = Last<N>::R, NS = Last<N>::NS;
+#pragma unroll
+  for (int b = 0; b < kE / R; ++b)
+#pragma unroll
+    for (int t = 0; t < R; ++t) st(out_idx<N, R, NS>(p, b, t), v[b * R + t]);
+}
+
+template <typename T>
+__device__ __forceinline__ c_t<T> czero() {
+  c_t<T> z;
+  z.x = T(0);
+  z.y = T(0);
+  return z;
+}
+
+// ---- shapes of the workgroups -----------------------------------------------
+// last-axis passes: rows per workgroup (even: two rows share a complex line)
+__host__ __device__ constexpr int z_rows(int n, int dim) {
+  return dim == 3 ? (n >= 512 ? 2 : 1024 / n) : (n >= 512 ? 4 : 2048 / n);
+}
+// first-axis projection pass: k-points per workgroup
+__host__ __device__ constexpr int x_kpts(int n, int dim) {
+  return dim == 3 ? (n >= 512 ? 1 : 512 / n) : (n >= 512 ? 2 : 1024 / n);
+}
+// strided passes: lines per workgroup (a run of 256 bytes)
+template <typename T>
+constexpr int s_lines() {
+  return 256 / (int)sizeof(c_t<T>);
+}
+
+// ---- strided c2c (middle axis, or first axis without projection) ----------
+// data[(o * N + m) * inner + w]: workgroup (w-tile, o) transforms lines
+// w0 .. w0 + L - 1 of plane o
+template <typename T, int N, int SG>
+__global__ __launch_bounds__(s_lines<T>() * (N / kE)) void k_strided_fast(
+    c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
+  constexpr int L = s_lines<T>(), LS = L + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
+  const int l = threadIdx.x % L, p = threadIdx.x / L;
+  const int64_t w = (int64_t)blockIdx.x * L + l;
+  const bool valid = w < inner;
+  c_t<T>* base = data + (int64_t)blockIdx.y * N * inner + w;
+  c_t<T> v[kE];
+  load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
+  Passes<T, N, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
+  if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
+}
+
+// ---- first axis: forward c2c, projection, backward c2c --------------------
+template <typename T>
+struct XArgs {
+  c_t<T>* data;
+  int64_t plane;  // k-points per first-axis index
+  const c_t<T>* tw;
+  int kind;
+  int allow_even;
+  int res[3];
+  int hlast;
+  double L[3];
+  T norm;
+};
+
+template <typename T, int N, int DIM>
+__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * (N / kE)) void k_axis0_fast(XArgs<T> a) {
+  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM), L = KP * NC, LS = L + 1, NT = L * (N / kE);
+  extern __shared__ __align__(16) unsigned char smem[];
+  c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
+  const int l = threadIdx.x % L, p = threadIdx.x / L;
+  const int64_t q0 = (int64_t)blockIdx.x * KP;
+  const int64_t inner = a.plane * NC;
+  const int64_t w = q0 * NC + l;
+  const bool valid = w < inner;
+  c_t<T>* base = a.data + w;
+  auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
+  c_t<T> v[kE];
+  load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
+  Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  __syncthreads();
+  // projection per wave vector (k0, q0 + kp)
+  for (int e = threadIdx.x; e < N * KP; e += NT) {
+    const int k0 = e / KP, kp = e - (e / KP) * KP;
+    const int64_t q = q0 + kp;
+    if (q >= a.plane) continue;
+    int c[3];
+    c[0] = k0;
+    if (DIM == 3) {
+      c[1] = (int)(q / a.hlast);
+      c[2] = (int)(q - (int64_t)c[1] * a.hlast);
+    } else {
+      c[1] = (int)q;
+    }
+    T xi[DIM];
+    c_t<T>* f = s + k0 * LS + kp * NC;
+    if (!unit_wave_vector<T, DIM>(c, a.res, a.L, a.allow_even, xi)) {
+#pragma unroll
+      for (int r = 0; r < NC; ++r) f[r] = czero<T>();
+      continue;
+    }
+    project_point<T, DIM>(f, 1, xi, a.kind, a.norm);
+  }
+  __syncthreads();
+  load_first<T, N>(v, p, [&](int m) { return at(m); });
+  __syncthreads();
+  Passes<T, N, +1, 0, 1>::run(v, p, a.tw, at);
+  if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
+}
+
+// ---- last axis: input stage, r2c ---------------------------------------------
+// Rows 2 rp and 2 rp + 1 of component c share complex line rp * NC + c (row
+// 2 rp in the real part), separated after the transform by Hermitian symmetry:
+//   X_a[k] = (Z[k] + conj Z[n-k]) / 2,  X_b[k] = (Z[k] - conj Z[n-k]) / 2i
+template <typename T>
+struct LArgs {
+  const T* in;
+  const T* rvec;
+  T* pout;
+  const double* scal;
+  const T* tan;
+  int tan_kind;
+  c_t<T>* spec;
+  T* out;
+  const T* dotv;
+  double* partials;
+  int64_t rows;
+  const c_t<T>* tw;
+};
+
+template <typename T, int N, int DIM, int IN>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r2c_fast(LArgs<T> a) {
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
+  constexpr int NT = L * (N / kE), NH = N / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
+  T* sr = reinterpret_cast<T*>(smem);
+  const int64_t row0 = (int64_t)blockIdx.x * RB;
+  if (IN == IN_PLAIN) {
+    const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
+    for (int e = threadIdx.x; e < RB * N * NC; e += NT) {
+      const int r = e / (N * NC), rem = e - r * (N * NC);
+      const int m = rem / NC, c = rem - m * NC;
+      sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = base + e < lim ? a.in[base + e] : T(0);
+    }
+  } else {
+    const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
+    for (int e = threadIdx.x; e < RB * N; e += NT) {
+      const int r = e / N, m = e & (N - 1);
+      const int64_t row = row0 + r;
+      T dp[NC];
+      if (row < a.rows) {
+        const int64_t vox = row * N + m;
+        T pv[NC];
+        if (IN == IN_TAN_PUPDATE) {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            pv[c] = -a.rvec[vox * NC + c] + beta * a.in[vox * NC + c];
+            a.pout[vox * NC + c] = pv[c];
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) pv[c] = a.in[vox * NC + c];
+        }
+        tangent_product<T, DIM>(a.tan_kind, a.tan, vox, pv, dp);
+      } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) dp[c] = T(0);
+      }
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
+    }
+  }
+  __syncthreads();
+  const int l = threadIdx.x % L, p = threadIdx.x / L;
+  auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
+  c_t<T> v[kE];
+  load_first<T, N>(v, p, [&](int m) { return at(m); });
+  __syncthreads();
+  Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  __syncthreads();
+  const int64_t obase = row0 * NH * NC, olim = a.rows * NH * NC;
+  for (int e = threadIdx.x; e < RB * NH * NC; e += NT) {
+    if (obase + e >= olim) break;
+    const int r = e / (NH * NC), rem = e - r * (NH * NC);
+    const int k = rem / NC, c = rem - k * NC;
+    const int ln = (r >> 1) * NC + c;
+    const c_t<T> z = s[k * LS + ln];
+    const c_t<T> zc = s[((N - k) & (N - 1)) * LS + ln];
+    c_t<T> X;
+    if (!(r & 1)) {
+      X.x = T(0.5) * (z.x + zc.x);
+      X.y = T(0.5) * (z.y - zc.y);
+    } else {
+      X.x = T(0.5) * (z.y + zc.y);
+      X.y = T(-0.5) * (z.x - zc.x);
+    }
+    a.spec[obase + e] = X;
+  }
+}
+
+// ---- last axis: c2r, output stage ---------------------------------------------
+// Half lines A (row 2 rp) and B (row 2 rp + 1) are completed by Hermitian
+// symmetry (imaginary parts of k = 0 and k = n/2 discarded) and combined into
+// Z = A + i B, whose backward transform holds row 2 rp in its real and row
+// 2 rp + 1 in its imaginary part.
+template <typename T, int N, int DIM, int OUT>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c2r_fast(LArgs<T> a) {
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
+  constexpr int NT = L * (N / kE), NH = N / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
+  const int64_t row0 = (int64_t)blockIdx.x * RB;
+  {
+    const int64_t base = row0 * NH * NC, lim = a.rows * NH * NC;
+    for (int e = threadIdx.x; e < RB * NH * NC; e += NT)
+      s[e] = base + e < lim ? a.spec[base + e] : czero<T>();
+  }
+  __syncthreads();
+  const int l = threadIdx.x % L, p = threadIdx.x / L;
+  const int rp = l / NC, cc = l - rp * NC;
+  c_t<T> v[kE];
+  load_first<T, N>(v, p, [&](int i) {
+    const int kk = i <= N / 2 ? i : N - i;
+    c_t<T> A = s[((2 * rp) * NH + kk) * NC + cc];
+    c_t<T> B = s[((2 * rp + 1) * NH + kk) * NC + cc];
+    if (kk == 0 || 2 * kk == N) {
+      A.y = T(0);
+      B.y = T(0);
+    }
+    if (i > N / 2) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    c_t<T> z;
+    z.x = A.x - B.y;
+    z.y = A.y + B.x;
+    return z;
+  });
+  __syncthreads();
+  auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
+  Passes<T, N, +1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  __syncthreads();
+  const T* sr = reinterpret_cast<const T*>(smem);
+  const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < RB * N * NC; e += NT) {
+    if (base + e >= lim) break;
+    const int r = e / (N * NC), rem = e - r * (N * NC);
+    const int m = rem / NC, c = rem - m * NC;
+    const T val = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
+    a.out[base + e] = val;
+    if (OUT == OUT_DOT) acc += (double)val * (double)a.dotv[base + e];
+  }
+  if (OUT == OUT_DOT) {
+    __syncthreads();
+    const double t = block_sum_any<NT>(acc, reinterpret_cast<double*>(smem));
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+  }
+}
+
+// ---- launchers ----------------------------------------------------------------
+template <typename T>
+const c_t<T>* tw_of(const FftPlan& p, int d) {
+  return reinterpret_cast<const c_t<T>*>(sizeof(T) == 8 ? p.tw64[d] : p.tw32[d]);
+}
+
+template <int N, int DIM>
+constexpr size_t lines_lds(size_t csz) {
+  return (size_t)N * (z_rows(N, DIM) / 2 * DIM * DIM + 1) * csz;
+}
+
+template <typename T, int N, int DIM>
+int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
+  LArgs<T> a{};
+  a.in = reinterpret_cast<const T*>(op.in);
+  a.rvec = reinterpret_cast<const T*>(op.rvec);
+  a.pout = reinterpret_cast<T*>(op.pout);
+  a.scal = op.scal;
+  a.tan = reinterpret_cast<const T*>(op.tangent);
+  a.tan_kind = op.tan_kind;
+  a.spec = reinterpret_cast<c_t<T>*>(spec);
+  a.rows = p.N / N;
+  a.tw = tw_of<T>(p, DIM - 1);
+  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / kE);
+  const size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
+  const dim3 grid((unsigned)((a.rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)));
+  switch (op.in_mode) {
+    case IN_PLAIN: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_PLAIN>), grid, dim3(NT), lds, stream(), a); break;
+    case IN_TAN: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_TAN>), grid, dim3(NT), lds, stream(), a); break;
+    default: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_TAN_PUPDATE>), grid, dim3(NT), lds, stream(), a); break;
+  }
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <typename T, int N, int DIM>
+int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
+  LArgs<T> a{};
+  a.spec = reinterpret_cast<c_t<T>*>(spec);
+  a.out = reinterpret_cast<T*>(op.out);
+  a.dotv = reinterpret_cast<const T*>(op.dotv);
+  a.partials = reinterpret_cast<double*>(p.partials);
+  a.rows = p.N / N;
+  a.tw = tw_of<T>(p, DIM - 1);
+  constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
+  constexpr int NT = RB / 2 * NC * (N / kE);
+  const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(c_t<T>);
+  size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
+  if (tile > lds) lds = tile;
+  if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
+  const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
+  if (op.out_mode == OUT_DOT)
+    hipLaunchKernelGGL((k_c2r_fast<T, N, DIM, OUT_DOT>), grid, dim3(NT), lds, stream(), a);
+  else
+    hipLaunchKernelGGL((k_c2r_fast<T, N, DIM, OUT_PLAIN>), grid, dim3(NT), lds, stream(), a);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <typename T, int N>
+int strided_n(FftPlan& p, int ax, void* spec, int sign) {
+  int64_t inner = p.nc;
+  for (int d = ax + 1; d < p.dim; ++d) inner *= p.hres[d];
+  int64_t outer = 1;
+  for (int d = 0; d < ax; ++d) outer *= p.hres[d];
+  constexpr int L = s_lines<T>();
+  constexpr int NT = L * (N / kE);
+  const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
+  const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
+  c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_strided_fast<T, N, -1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+  else
+    hipLaunchKernelGGL((k_strided_fast<T, N, +1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <typename T, int N, int DIM>
+int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
+  XArgs<T> a{};
+  a.data = reinterpret_cast<c_t<T>*>(spec);
+  a.plane = p.kpts / p.res[0];
+  a.tw = tw_of<T>(p, 0);
+  a.kind = pj.kind;
+  a.allow_even = pj.allow_even;
+  for (int d = 0; d < 3; ++d) {
+    a.res[d] = p.res[d];
+    a.L[d] = pj.L[d];
+  }
+  a.hlast = p.hres[DIM - 1];
+  a.norm = (T)pj.norm;
+  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * (N / kE);
+  const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
+  const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
+  hipLaunchKernelGGL((k_axis0_fast<T, N, DIM>), grid, dim3(NT), lds, stream(), a);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+// dispatch on the axis length
+#define MUGPU_FAST_SWITCH(n, CALL) \
+  switch (n) {                     \
+    case 16: return CALL(16);      \
+    case 32: return CALL(32);      \
+    case 64: return CALL(64);      \
+    case 128: return CALL(128);    \
+    case 256: return CALL(256);    \
+    case 512: return CALL(512);    \
+    default: break;                \
+  }
+
+template <typename T, int DIM>
+int r2c_any(FftPlan& p, const OpSpec& op, void* spec) {
+#define C_(n) r2c_n<T, n, DIM>(p, op, spec)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], C_)
+#undef C_
+  return MUGPU_EARG;
+}
+template <typename T, int DIM>
+int c2r_any(FftPlan& p, const OpSpec& op, void* spec) {
+#define C_(n) c2r_n<T, n, DIM>(p, op, spec)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], C_)
+#undef C_
+  return MUGPU_EARG;
+}
+template <typename T>
+int strided_any(FftPlan& p, int ax, void* spec, int sign) {
+#define C_(n) strided_n<T, n>(p, ax, spec, sign)
+  MUGPU_FAST_SWITCH(p.res[ax], C_)
+#undef C_
+  return MUGPU_EARG;
+}
+template <typename T, int DIM>
+int axis0_any(FftPlan& p, const ProjParams& pj, void* spec) {
+#define C_(n) axis0_n<T, n, DIM>(p, pj, spec)
+  MUGPU_FAST_SWITCH(p.res[0], C_)
+#undef C_
+  return MUGPU_EARG;
+}
+
+}  // namespace
+
+int fast_rows_per_block(int n, int dim) { return z_rows(n, dim); }
+
+int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec) {
+  if (prec == MUGPU_FP64) return p.dim == 3 ? r2c_any<double, 3>(p, op, spec) : r2c_any<double, 2>(p, op, spec);
+  return p.dim == 3 ? r2c_any<float, 3>(p, op, spec) : r2c_any<float, 2>(p, op, spec);
+}
+int fast_c2r(FftPlan& p, int prec, const OpSpec& op, void* spec) {
+  if (prec == MUGPU_FP64) return p.dim == 3 ? c2r_any<double, 3>(p, op, spec) : c2r_any<double, 2>(p, op, spec);
+  return p.dim == 3 ? c2r_any<float, 3>(p, op, spec) : c2r_any<float, 2>(p, op, spec);
+}
+int fast_strided(FftPlan& p, int prec, int ax, void* spec, int sign) {
+  if (prec == MUGPU_FP64) return strided_any<double>(p, ax, spec, sign);
+  return strided_any<float>(p, ax, spec, sign);
+}
+int fast_axis0_project(FftPlan& p, int prec, const ProjParams& pj, void* spec) {
+  if (prec == MUGPU_FP64) return p.dim == 3 ? axis0_any<double, 3>(p, pj, spec) : axis0_any<double, 2>(p, pj, spec);
+  return p.dim == 3 ? axis0_any<float, 3>(p, pj, spec) : axis0_any<float, 2>(p, pj, spec);
+}
+
+}  // namespace mugpu

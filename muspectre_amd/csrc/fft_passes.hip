@@ -19,51 +19,9 @@
 #include <cstdio>
 
 #include "mugpu_internal.hpp"
+#include "op_stages.hpp"
 
 namespace mugpu {
-
-template <typename T, int DIM>
-__device__ __forceinline__ void tangent_product(int kind, const T* __restrict__ tan,
-                                                int64_t v, const T* p, T* dp) {
-  constexpr int NC = DIM * DIM;
-  if (kind == TAN_ISO) {
-    const T lam = tan[2 * v], mu = tan[2 * v + 1];
-    T tr = 0;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) tr += p[i * (DIM + 1)];
-#pragma unroll
-    for (int j = 0; j < DIM; ++j)
-#pragma unroll
-      for (int i = 0; i < DIM; ++i)
-        dp[i + DIM * j] = (i == j ? lam * tr : T(0)) + mu * (p[i + DIM * j] + p[j + DIM * i]);
-  } else if (kind == TAN_HEP) {
-    const T* M = tan + v * (DIM * DIM * DIM);
-    T s[DIM];
-#pragma unroll
-    for (int n = 0; n < DIM; ++n) {
-      T a = 0;
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) a += p[j + DIM * n];
-      s[n] = a;
-    }
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      T a = 0;
-#pragma unroll
-      for (int n = 0; n < DIM; ++n) a += M[q + NC * n] * s[n];
-      dp[q] = a;
-    }
-  } else {
-    const T* K = tan + v * (NC * NC);
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      T a = 0;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) a += K[r + NC * c] * p[c];
-      dp[r] = a;
-    }
-  }
-}
 
 template <typename T>
 struct ZArgs {
@@ -336,13 +294,6 @@ struct PArgs {
   T norm;
 };
 
-template <typename T>
-__device__ __forceinline__ T fft_freq(int c, int n, double L) {
-  // numpy fftfreq(n)*n order divided by the length (src/fft/fft_utils.cc:41-56)
-  const int npos = (n - 1) / 2 + 1;
-  return T((double)(c < npos ? c : c - n) / L);
-}
-
 template <typename T, int NT, int DIM>
 __global__ __launch_bounds__(NT) void k_axis0_project(PArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -379,18 +330,8 @@ __global__ __launch_bounds__(NT) void k_axis0_project(PArgs<T> a) {
       c[1] = (int)q;
     }
     T xi[DIM];
-    T n2 = 0;
-    bool zero = true;
-    bool nyq = false;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      xi[d] = fft_freq<T>(c[d], a.res[d], a.L[d]);
-      n2 += xi[d] * xi[d];
-      if (c[d] != 0) zero = false;
-      if (a.allow_even && (a.res[d] % 2 == 0) && 2 * c[d] == a.res[d]) nyq = true;
-    }
     c_t<T>* f = s + (wk * NC) * a.ls + k0;
-    if (zero || nyq) {
+    if (!unit_wave_vector<T, DIM>(c, a.res, a.L, a.allow_even, xi)) {
 #pragma unroll
       for (int r = 0; r < NC; ++r) {
         f[r * a.ls].x = T(0);
@@ -398,70 +339,7 @@ __global__ __launch_bounds__(NT) void k_axis0_project(PArgs<T> a) {
       }
       continue;
     }
-    const T inv = T(1) / sqrt(n2);
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) xi[d] *= inv;
-    T fr[NC], fi[NC];
-#pragma unroll
-    for (int r = 0; r < NC; ++r) {
-      const c_t<T> v = f[r * a.ls];
-      fr[r] = v.x;
-      fi[r] = v.y;
-    }
-    if (a.kind == MUGPU_PROJ_SMALL_STRAIN) {
-      // G f = (xi_i a_j + xi_i b_j + xi_j a_i + xi_j b_i)/2 - xi_i xi_j s,
-      // a = f xi, b = f^T xi, s = xi^T f xi
-      T ar[DIM], br[DIM], ai[DIM], bi[DIM];
-      T sr_ = 0, si_ = 0;
-#pragma unroll
-      for (int i = 0; i < DIM; ++i) {
-        T a1 = 0, b1 = 0, a2 = 0, b2 = 0;
-#pragma unroll
-        for (int m = 0; m < DIM; ++m) {
-          a1 += fr[i + DIM * m] * xi[m];
-          a2 += fi[i + DIM * m] * xi[m];
-          b1 += fr[m + DIM * i] * xi[m];
-          b2 += fi[m + DIM * i] * xi[m];
-        }
-        ar[i] = a1;
-        ai[i] = a2;
-        br[i] = b1;
-        bi[i] = b2;
-        sr_ += xi[i] * a1;
-        si_ += xi[i] * a2;
-      }
-#pragma unroll
-      for (int j = 0; j < DIM; ++j)
-#pragma unroll
-        for (int i = 0; i < DIM; ++i) {
-          const T gr = T(0.5) * (xi[i] * ar[j] + xi[i] * br[j] + xi[j] * ar[i] + xi[j] * br[i]) -
-                       xi[i] * xi[j] * sr_;
-          const T gi = T(0.5) * (xi[i] * ai[j] + xi[i] * bi[j] + xi[j] * ai[i] + xi[j] * bi[i]) -
-                       xi[i] * xi[j] * si_;
-          c_t<T> g;
-          g.x = a.norm * gr;
-          g.y = a.norm * gi;
-          f[(i + DIM * j) * a.ls] = g;
-        }
-    } else {
-      // f_ij <- norm (sum_k f_ik xi_k) xi_j
-#pragma unroll
-      for (int i = 0; i < DIM; ++i) {
-        T s1 = 0, s2 = 0;
-#pragma unroll
-        for (int k = 0; k < DIM; ++k) {
-          s1 += fr[i + DIM * k] * xi[k];
-          s2 += fi[i + DIM * k] * xi[k];
-        }
-#pragma unroll
-        for (int j = 0; j < DIM; ++j) {
-          c_t<T> g;
-          g.x = a.norm * (s1 * xi[j]);
-          g.y = a.norm * (s2 * xi[j]);
-          f[(i + DIM * j) * a.ls] = g;
-        }
-      }
-    }
+    project_point<T, DIM>(f, a.ls, xi, a.kind, a.norm);
   }
   __syncthreads();
   lds_fft<T, NT>(s, W, a.ls, a.ap, a.tw, +1);
@@ -579,20 +457,25 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     return lines * len <= (int64_t)nt * kEPT &&
            (size_t)lines * odd_stride(len) * 16 <= lds;
   };
+  for (int d = 0; d < dim; ++d) p.fastax[d] = fast_size(res[d]);
   const int n = res[dim - 1];
-  if (!fits((nc + 1) / 2, n, kNT)) {
-    set_error("last axis of %d points exceeds the supported line length", n);
-    return MUGPU_EARG;
-  }
   const int64_t rows = p.N / n;
-  int rb = 1;
-  while (rb < rows && rb < 64 && fits(((rb + 1) * nc + 1) / 2, n, kNT) &&
-         fits((int64_t)(rb + 1) * (n / 2 + 1) * nc, 1, kNT * 64))
-    ++rb;
-  p.zrb = rb;
-  p.nzblocks = (rows + rb - 1) / rb;
+  if (p.fastax[dim - 1]) {
+    p.zrb = fast_rows_per_block(n, dim);
+  } else {
+    if (!fits((nc + 1) / 2, n, kNT)) {
+      set_error("last axis of %d points exceeds the supported line length", n);
+      return MUGPU_EARG;
+    }
+    int rb = 1;
+    while (rb < rows && rb < 64 && fits(((rb + 1) * nc + 1) / 2, n, kNT) &&
+           fits((int64_t)(rb + 1) * (n / 2 + 1) * nc, 1, kNT * 64))
+      ++rb;
+    p.zrb = rb;
+  }
+  p.nzblocks = (rows + p.zrb - 1) / p.zrb;
   // strided middle axis: up to 16 lines (256 B rows in FP64)
-  if (dim == 3) {
+  if (dim == 3 && !p.fastax[1]) {
     const int ny = res[1];
     int W = 16;
     while (W > 1 && !fits(W, ny, kNT)) W /= 2;
@@ -603,7 +486,7 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     p.yW = W;
   }
   const int nx = res[0];
-  {
+  if (!p.fastax[0]) {
     int W = 16;
     while (W > 1 && !fits(W, nx, kNT)) W /= 2;
     if (!fits(W, nx, kNT)) {
@@ -613,7 +496,7 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     p.xW = W;
   }
   // fused first-axis pass: whole k-points (nc lines each)
-  {
+  if (!p.fastax[0]) {
     int NT = 256;
     while (NT < 1024 && !fits(nc, nx, NT)) NT *= 2;
     if (!fits(nc, nx, NT)) {
@@ -646,8 +529,19 @@ const c_t<T>* tw_of(const FftPlan& p, int d) {
   return reinterpret_cast<const c_t<T>*>(sizeof(T) == 8 ? p.tw64[d] : p.tw32[d]);
 }
 
+template <typename T>
+constexpr int prec_of() {
+  return sizeof(T) == 8 ? MUGPU_FP64 : MUGPU_FP32;
+}
+
 template <typename T, int DIM>
 int launch_r2c(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
+  if (p.fastax[DIM - 1]) {
+    timing_begin(MUGPU_K_LAST_R2C);
+    const int rc = fast_r2c(p, prec_of<T>(), op, spec);
+    timing_end(MUGPU_K_LAST_R2C);
+    return rc;
+  }
   ZArgs<T> a{};
   a.in = reinterpret_cast<const T*>(op.in);
   a.rvec = reinterpret_cast<const T*>(op.rvec);
@@ -678,6 +572,12 @@ int launch_r2c(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
 
 template <typename T, int DIM>
 int launch_c2r(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
+  if (p.fastax[DIM - 1]) {
+    timing_begin(MUGPU_K_LAST_C2R);
+    const int rc = fast_c2r(p, prec_of<T>(), op, spec);
+    timing_end(MUGPU_K_LAST_C2R);
+    return rc;
+  }
   ZArgs<T> a{};
   a.spec = spec;
   a.out = reinterpret_cast<T*>(op.out);
@@ -716,6 +616,13 @@ int launch_strided(FftPlan& p, int ax, int W, c_t<T>* spec, int sign) {
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   const int n = p.res[ax];
   if (n == 1) return MUGPU_OK;
+  if (p.fastax[ax]) {
+    const int kc = ax == 0 ? MUGPU_K_AXIS0 : MUGPU_K_MIDDLE;
+    timing_begin(kc);
+    const int rc = fast_strided(p, prec_of<T>(), ax, spec, sign);
+    timing_end(kc);
+    return rc;
+  }
   a.inner = inner;
   a.W = W;
   a.ls = odd_stride(n);
@@ -734,6 +641,12 @@ int launch_strided(FftPlan& p, int ax, int W, c_t<T>* spec, int sign) {
 
 template <typename T, int DIM>
 int launch_project(FftPlan& p, const ProjParams& pj, c_t<T>* spec) {
+  if (p.fastax[0]) {
+    timing_begin(MUGPU_K_AXIS0);
+    const int rc = fast_axis0_project(p, prec_of<T>(), pj, spec);
+    timing_end(MUGPU_K_AXIS0);
+    return rc;
+  }
   PArgs<T> a{};
   a.data = spec;
   a.plane = p.kpts / p.res[0];

@@ -11,8 +11,9 @@
 //   139-148,313-350); Kirchhoff -> PK1 as coded (:374-410)
 //
 // The tangent is written in the representation the cell chose: ISO
-// (lambda, mu), HEP (dim^3, j-independent Kirchhoff->PK1 tangent) or FULL
-// (the reference's nc x nc matrix).
+// (lambda, mu), HEP (dim^3, j-independent Kirchhoff->PK1 tangent), FULL
+// (the reference's nc x nc matrix) or SYM (its upper triangle), in the
+// voxel-blocked layout of tan_off().
 #include <cmath>
 
 #include "mugpu_internal.hpp"
@@ -297,7 +298,7 @@ __device__ void hep1_consistent_tangent(const double* Fi, const double* bes, con
               s += g[m + D * n] * V[i + D * m] * V[j + D * n] * V[k + D * m] * V[l + D * n];
           Lc[i + D * j] = s;
         }
-      double* DLc = K + NC * (k + D * l);  // K is scratch for DL here
+      double* DLc = K + NC * (k + D * l);  // K holds DL until the last loop
 #pragma unroll
       for (int r = 0; r < NC; ++r) {
         double s = 0;
@@ -343,15 +344,24 @@ struct MatArgs {
   const double *Fp_old, *be_old, *ep_old;
 };
 
+// one entry K(r, c) of an nc x nc tangent in the FULL or SYM representation
+template <int D>
+__device__ __forceinline__ void put_K(const MatArgs& a, int64_t v, int r, int c, double val) {
+  constexpr int NC = D * D;
+  if (a.tan_kind == TAN_SYM) {
+    if (r <= c) a.tan[tan_off(v, sym_index(r, c, NC), NC * (NC + 1) / 2)] = val;
+  } else {
+    a.tan[tan_off(v, r + NC * c, NC * NC)] = val;
+  }
+}
+
 // write an isotropic tangent (lambda, mu) in the cell's representation
 template <int D>
 __device__ void store_iso(const MatArgs& a, int64_t v, double lam, double mu) {
-  constexpr int NC = D * D;
   if (a.tan_kind == TAN_ISO) {
-    a.tan[2 * v] = lam;
-    a.tan[2 * v + 1] = mu;
+    a.tan[tan_off(v, 0, 2)] = lam;
+    a.tan[tan_off(v, 1, 2)] = mu;
   } else {
-    double* K = a.tan + v * NC * NC;
 #pragma unroll
     for (int l = 0; l < D; ++l)
 #pragma unroll
@@ -359,7 +369,7 @@ __device__ void store_iso(const MatArgs& a, int64_t v, double lam, double mu) {
 #pragma unroll
         for (int j = 0; j < D; ++j)
 #pragma unroll
-          for (int i = 0; i < D; ++i) K[(i + D * j) + NC * (k + D * l)] = iso_C<D>(lam, mu, i, j, k, l);
+          for (int i = 0; i < D; ++i) put_K<D>(a, v, i + D * j, k + D * l, iso_C<D>(lam, mu, i, j, k, l));
   }
 }
 
@@ -374,7 +384,6 @@ __device__ void pk1_from_pk2_store(const MatArgs& a, int64_t v, const double* F,
 #pragma unroll
   for (int q = 0; q < NC; ++q) P[q] = Pl[q];
   if (!a.need_tangent) return;
-  double* K = a.tan + v * NC * NC;
   for (int n = 0; n < D; ++n)
     for (int j = 0; j < D; ++j)
       for (int m = 0; m < D; ++m)
@@ -382,7 +391,7 @@ __device__ void pk1_from_pk2_store(const MatArgs& a, int64_t v, const double* F,
           double k = (i == j) ? S[m + D * n] : 0.0;
           for (int r = 0; r < D; ++r)
             for (int s = 0; s < D; ++s) k += F[i + D * r] * iso_C<D>(lam, mu, r, m, n, s) * F[j + D * s];
-          K[(i + D * m) + NC * (j + D * n)] = k;
+          put_K<D>(a, v, i + D * m, j + D * n, k);
         }
 }
 
@@ -522,7 +531,10 @@ __global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
     double Pl[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) Pl[c] = a.P[v * NC + c];
-    hep1_consistent_tangent<D>(Fi, bes, C, Pl, a.tan + v * NC * NC);
+    double K[NC * NC];
+    hep1_consistent_tangent<D>(Fi, bes, C, Pl, K);
+    for (int c = 0; c < NC; ++c)
+      for (int r = 0; r < NC; ++r) put_K<D>(a, v, r, c, K[r + NC * c]);
     return;
   }
   // Kirchhoff tangent C(r,m,n,s), then M(i,m,n) = sum_r Finv(i,r) sum_s C(r,m,n,s)
@@ -562,15 +574,13 @@ __global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
         M[i + D * m + D * D * n] = acc;
       }
   if (a.tan_kind == TAN_HEP) {
-    double* out = a.tan + v * (D * D * D);
 #pragma unroll
-    for (int e = 0; e < D * D * D; ++e) out[e] = M[e];
+    for (int e = 0; e < D * D * D; ++e) a.tan[tan_off(v, e, D * D * D)] = M[e];
   } else {
-    double* K = a.tan + v * NC * NC;
     for (int n = 0; n < D; ++n)
       for (int j = 0; j < D; ++j)
         for (int m = 0; m < D; ++m)
-          for (int i = 0; i < D; ++i) K[(i + D * m) + NC * (j + D * n)] = M[i + D * m + D * D * n];
+          for (int i = 0; i < D; ++i) put_K<D>(a, v, i + D * m, j + D * n, M[i + D * m + D * D * n]);
   }
 }
 

@@ -33,17 +33,39 @@ void timing_begin(int kclass);
 void timing_end(int kclass);
 
 // ---- tangent representations of the directional-stiffness product --------
-// FULL: the reference's stored K, nc*nc per voxel, K(r,c) at r + nc*c
-//       (src/common/T4_map_proxy.hh:83-105)
+// FULL: the reference's stored K, nc*nc entries per voxel, entry r + nc*c is
+//       K(r, c) (src/common/T4_map_proxy.hh:83-105)
 // ISO : (lambda, mu) per voxel; dP = lambda tr(dF) I + mu (dF + dF^T), the
 //       contraction with C = lambda Itrac + 2 mu Isymm
 // HEP : dim^3 per voxel, M(i,m,n) at i + d m + d^2 n, for the j-independent
 //       Kirchhoff->PK1 tangent K(i,m,j,n) = M(i,m,n)
 //       (src/materials/materials_toolbox.hh:394-406)
-enum TangentKind { TAN_FULL = 0, TAN_ISO = 1, TAN_HEP = 2 };
-inline int tangent_stride(int kind, int dim) {
+// SYM : the upper triangle r <= c of a major-symmetric K (finite-strain
+//       hyperelastic laws and the consistent elasto-plastic tangent),
+//       nc (nc + 1) / 2 entries, row by row
+//
+// Storage is voxel-blocked: blocks of kTanBlock consecutive voxels, entry
+// major inside a block, so that lanes on consecutive voxels read one
+// contiguous run per entry (coalesced) while a voxel's entries stay in one
+// block.
+enum TangentKind { TAN_FULL = 0, TAN_ISO = 1, TAN_HEP = 2, TAN_SYM = 3 };
+constexpr int kTanBlock = 64;
+__host__ __device__ inline int tangent_stride(int kind, int dim) {
   const int nc = dim * dim;
-  return kind == TAN_FULL ? nc * nc : kind == TAN_ISO ? 2 : dim * dim * dim;
+  return kind == TAN_FULL ? nc * nc
+                          : kind == TAN_ISO ? 2 : kind == TAN_HEP ? dim * dim * dim : nc * (nc + 1) / 2;
+}
+// number of values allocated for a tangent over npix voxels
+inline int64_t tangent_values(int kind, int dim, int64_t npix) {
+  return (npix + kTanBlock - 1) / kTanBlock * kTanBlock * tangent_stride(kind, dim);
+}
+// offset of entry q of voxel v for a stride of S entries per voxel
+__host__ __device__ inline int64_t tan_off(int64_t v, int q, int S) {
+  return (v / kTanBlock) * ((int64_t)S * kTanBlock) + (int64_t)q * kTanBlock + (v % kTanBlock);
+}
+// packed index of K(r, c), r <= c, in the SYM representation of an n x n matrix
+__host__ __device__ constexpr int sym_index(int r, int c, int n) {
+  return r * n - r * (r - 1) / 2 + (c - r);
 }
 
 // device scalar block used by the CG kernels
@@ -65,6 +87,7 @@ struct FftPlan {
   int xNT = 256;   // threads of the fused pass
   void* partials = nullptr;  // double per z-pass workgroup
   int64_t nzblocks = 0;
+  bool fastax[3] = {false, false, false};  // compile-time-length passes (fft_fast.hip)
 };
 
 int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec);
@@ -103,6 +126,14 @@ int run_operator(FftPlan& p, int prec, const OpSpec& op, const ProjParams& pj,
 // engine transforms (no projection)
 int run_r2c(FftPlan& p, int prec, const void* in, void* spectrum);
 int run_c2r(FftPlan& p, int prec, void* spectrum, void* out);
+
+// passes for axis lengths 16, 32, ..., 512 (fft_fast.hip)
+bool fast_size(int n);
+int fast_rows_per_block(int n, int dim);
+int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec);
+int fast_c2r(FftPlan& p, int prec, const OpSpec& op, void* spec);
+int fast_strided(FftPlan& p, int prec, int ax, void* spec, int sign);
+int fast_axis0_project(FftPlan& p, int prec, const ProjParams& pj, void* spec);
 
 // ---- reductions and vector kernels (cg.hip) --------------------------------
 int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,

@@ -84,21 +84,30 @@ def _sphere_mask(res, r):
     return sum((gi - ci) ** 2 for gi, ci in zip(g, c)) < r * r
 
 
+# (name, resolutions, lengths, formulation, family in, family out, E in,
+#  E out[, allow_even]); power-of-two grids run the compile-time-length FFT
+# passes and need the even-grid policy
 CASES = [
     ("2d_small_le1", [51, 51], [7., 5.], 1, 1, 1, 10e9, 70e9),
     ("3d_small_le1", [9, 7, 11], [1., 1., 1.], 1, 1, 1, 100e9, 10e9),
     ("2d_finite_le1", [7, 9], [1., 1.], 0, 1, 1, 210e9, 70e9),
     ("3d_finite_le1", [5, 7, 5], [1., 1., 1.], 0, 1, 1, 210e9, 70e9),
+    ("2d_small_le1_pow2", [64, 128], [1., 2.], 1, 1, 1, 10e9, 70e9, 1),
+    ("2d_finite_le1_pow2", [32, 512], [1., 1.], 0, 1, 1, 210e9, 70e9, 1),
+    ("3d_small_le1_pow2", [32, 16, 64], [1., 1., 1.], 1, 1, 1, 100e9, 10e9, 1),
+    ("3d_finite_le1_pow2", [16, 32, 16], [1., 1.3, .7], 0, 1, 1, 210e9, 70e9, 1),
+    ("3d_finite_le1_mixed", [16, 9, 32], [1., 1., 1.], 0, 1, 1, 210e9, 70e9, 1),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
 def test_stress_tangent_and_directional_stiffness(capi, oracle, case):
-    name, res, L, form, f_in, f_out, Ein, Eout = case
+    name, res, L, form, f_in, f_out, Ein, Eout = case[:8]
+    even = case[8] if len(case) > 8 else 0
     d = len(res)
     nc = d * d
-    oc = oracle.Cell(res, L, form)
-    gc = GpuCell(capi, res, L, form)
+    oc = oracle.Cell(res, L, form, allow_even=bool(even))
+    gc = GpuCell(capi, res, L, form, allow_even=even)
     _two_phase(oc, gc, _sphere_mask(res, min(res) / 3), f_in, f_out, Ein, Eout, .33)
     oc.initialise()
     capi.check(gc.lib.mugpu_cell_initialise(gc.h))
@@ -147,11 +156,12 @@ def _gpu_cg(gc, rhs, tol, maxiter):
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: c[0])
 def test_cg_solve_vs_oracle(capi, oracle, case):
-    name, res, L, form, f_in, f_out, Ein, Eout = case
+    name, res, L, form, f_in, f_out, Ein, Eout = case[:8]
+    even = case[8] if len(case) > 8 else 0
     d = len(res)
     nc = d * d
-    oc = oracle.Cell(res, L, form)
-    gc = GpuCell(capi, res, L, form)
+    oc = oracle.Cell(res, L, form, allow_even=bool(even))
+    gc = GpuCell(capi, res, L, form, allow_even=even)
     _two_phase(oc, gc, _sphere_mask(res, min(res) / 3), f_in, f_out, Ein, Eout, .3)
     oc.initialise()
     capi.check(gc.lib.mugpu_cell_initialise(gc.h))
