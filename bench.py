@@ -96,7 +96,9 @@ class Workload:
             # geometric stress with the near-deviatoric-free return-mapping
             # modulus of a large plastic step) on which CG diverges
             self.newton_maxiter = 1
-            self.B_alg = 1728  # 9*(6*8+9*8) + 648: full 81-double tangent
+            # 9*(6*8 + 9*8) + 360: the tangent is major-symmetric, stored as
+            # its 45-entry upper triangle
+            self.T = 360
         elif w == "c2":
             self.n = args.size or 128
             self.res = [self.n] * 3
@@ -111,7 +113,7 @@ class Workload:
             self.loads = cases
             self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 4000
             self.equil_tol = 1e-10
-            self.B_alg = 1096  # 9*(6*8+9*8) + 16: (lambda, mu) per voxel
+            self.T = 16  # (lambda, mu) per voxel
         elif w == "c5":
             self.n = args.size or 512
             self.res = [self.n] * 3
@@ -125,7 +127,7 @@ class Workload:
             self.loads = [[e]]
             self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 4000
             self.equil_tol = 1e-10
-            self.B_alg = 1096
+            self.T = 16
         else:
             self.n = args.size or 51
             self.res = [self.n, self.n]
@@ -136,13 +138,16 @@ class Workload:
             self.loads = [[np.array([[0, 0], [0, .03]])]]
             self.newton_tol, self.cg_tol, self.maxiter = 1e-5, 1e-8, 401
             self.equil_tol = 2.0
-            self.B_alg = 4 * (6 * 8 + 9 * 8) + 16
+            self.T = 16
         self.N = int(np.prod(self.res))
-        if self.prec == "fp32":
-            # FP32 operator fields and CG vectors, FP32 tangent copy
-            nc = len(self.res) ** 2
-            T = {"c3": 81 * 4, "c2": 8, "c5": 8, "c1": 8}[self.name]
-            self.B_alg = nc * (6 * 4 + 9 * 4) + T
+        # algorithmic bytes per voxel per CG iteration: nc (6 s_f + 9 s_v) + T,
+        # six sweeps of operator-precision fields (K:p input, two spectrum
+        # round trips, Ap) and nine of CG vectors, plus the tangent; the FP32
+        # form keeps operator fields, CG vectors and the tangent copy in FP32
+        nc = len(self.res) ** 2
+        s = 8 if self.prec == "fp64" else 4
+        self.T_prec = self.T * s // 8
+        self.B_alg = nc * (6 * s + 9 * s) + self.T_prec
         if not hasattr(self, "newton_maxiter"):
             self.newton_maxiter = 0
         self.converged = None
@@ -194,8 +199,19 @@ class Workload:
 
 # -------------------------------------------------------- kernel accounting
 K_IDS = {"r2c": 0, "middle": 1, "axis0": 2, "c2r": 3, "material": 4, "update": 5}
-K_NAMES = {0: "k_last_r2c", 1: "k_strided (middle axis)", 2: "k_axis0_project",
-           3: "k_last_c2r", 4: "material", 5: "k_cg_update"}
+FAST = (16, 32, 64, 128, 256, 512)
+
+
+def kernel_name(wl, k):
+    """Symbol of the kernel behind a timing class for this grid: axis
+    lengths 16..512 run the compile-time-length passes."""
+    last, first = wl.res[-1], wl.res[0]
+    mid = wl.res[1] if len(wl.res) == 3 else None
+    return {0: "k_r2c_fast" if last in FAST else "k_last_r2c",
+            1: "k_strided_fast (middle axis)" if mid in FAST else "k_strided (middle axis)",
+            2: "k_axis0_fast" if first in FAST else "k_axis0_project",
+            3: "k_c2r_fast" if last in FAST else "k_last_c2r",
+            4: "material", 5: "k_cg_update"}[k]
 
 
 def kernel_bytes(wl, kclass, prec):
@@ -209,6 +225,13 @@ def kernel_bytes(wl, kclass, prec):
     kpts = int(np.prod(herm))
     if kclass in (1, 2):
         return 2 * kpts * nc * 2 * s
+    if kclass == 0:
+        # r and p read, p written (p = -r + beta p), tangent read, half
+        # spectrum written
+        return 3 * wl.N * nc * s + wl.N * wl.T_prec + kpts * nc * 2 * s
+    if kclass == 3:
+        # half spectrum read, Ap written, p read for the dot product
+        return kpts * nc * 2 * s + 2 * wl.N * nc * s
     if kclass == 5:
         return 6 * wl.N * nc * s
     return None
@@ -291,9 +314,10 @@ def traffic_from_profiles(wl, kname):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 3 and parts[0] == wl.name + ":" + str(wl.n) and parts[1] == kname:
+        if (len(parts) >= 4 and parts[0] == wl.name + ":" + str(wl.n) and parts[1] == wl.prec
+                and parts[2] == kname.split()[0]):
             try:
-                return float(parts[2])
+                return float(parts[3])
             except ValueError:
                 return None
     return None
@@ -358,21 +382,16 @@ def main():
         print("bench: warmup {}: {} Newton, {} CG iterations".format(w, its, cg),
               file=sys.stderr, flush=True)
         del cell
-    kchoice = args.kernel
-    if kchoice == "auto":
-        kchoice = "middle" if len(wl.res) == 3 else "axis0"
-    kclass = K_IDS[kchoice]
     total = 0.0
     setup = 0.0
     tot_its = tot_cg = 0
-    launches_all = 0
-    kms_all = 0.0
+    stats = {k: [0, 0.0] for k in K_IDS.values()}
     for s in range(args.steps):
         t0 = time.perf_counter()
         cell = wl.make_cell(mu)
         _capi.check(lib.mugpu_synchronize())
         setup += time.perf_counter() - t0
-        _capi.check(lib.mugpu_timing_start(kclass))
+        _capi.check(lib.mugpu_timing_start(_capi.K_ALL))
         _capi.check(lib.mugpu_synchronize())
         t0 = time.perf_counter()
         its, cg = wl.solve(mu, cell)
@@ -381,8 +400,10 @@ def main():
         launches = ctypes.c_int64()
         kms = ctypes.c_double()
         _capi.check(lib.mugpu_timing_stop(ctypes.byref(launches), ctypes.byref(kms)))
-        launches_all += launches.value
-        kms_all += kms.value
+        for k in stats:
+            _capi.check(lib.mugpu_timing_class(k, ctypes.byref(launches), ctypes.byref(kms)))
+            stats[k][0] += launches.value
+            stats[k][1] += kms.value
         total += dt
         tot_its += its
         tot_cg += cg
@@ -390,16 +411,35 @@ def main():
               file=sys.stderr, flush=True)
         del cell
     value = wl.N * tot_cg / total
-    kb = kernel_bytes(wl, kclass, wl.prec)
+    # per-kernel-class durations (HIP events on the library stream) and the
+    # bandwidth of their algorithmic bytes
+    kernels = {}
+    for name, k in K_IDS.items():
+        n, ms = stats[k]
+        if not n:
+            continue
+        avg = ms / n / 1e3
+        kb = kernel_bytes(wl, k, wl.prec)
+        e = {"kernel": kernel_name(wl, k), "launches": n, "avg_launch_us": round(avg * 1e6, 2),
+             "share_of_wall": round(ms / 1e3 / total, 4)}
+        if kb:
+            e["bytes_per_launch"] = kb
+            e["achieved_GBs"] = round(kb / avg / 1e9, 1)
+            e["frac"] = round(kb / avg / 1e9 / HBM_PEAK_GBS, 4)
+        kernels[name] = e
+    kchoice = args.kernel
+    if kchoice == "auto":
+        # the dominant kernel: largest share of the run among the HBM passes
+        kchoice = max((n for n in kernels if "frac" in kernels[n]),
+                      key=lambda n: kernels[n]["share_of_wall"])
+    kclass = K_IDS[kchoice]
     roof = None
-    if launches_all and kb:
-        avg = kms_all / launches_all / 1e3
-        ach = kb / avg / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(wl, K_NAMES[kclass]),
-                "kernel": K_NAMES[kclass], "launches": launches_all,
-                "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": kb}
+    if kchoice in kernels and "frac" in kernels[kchoice]:
+        e = kernels[kchoice]
+        roof = {"bound": "hbm", "achieved": e["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": e["frac"], "traffic": traffic_from_profiles(wl, kernel_name(wl, kclass)),
+                "kernel": kernel_name(wl, kclass), "launches": e["launches"],
+                "avg_launch_us": e["avg_launch_us"], "bytes_per_launch": e["bytes_per_launch"]}
     fp32_error = None
     if wl.prec == "fp32":
         fp32_error = mixed_precision_error(mu, wl)
@@ -434,6 +474,7 @@ def main():
             "bytes_per_voxel_iter": wl.B_alg,
             "achieved": round(wl.B_alg * value / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(wl.B_alg * value / 1e9 / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
         "copy_kernel_GBs": round(copy_gbs, 1),
         "fp32_error": fp32_error,
         "cpu_baseline": cpu,

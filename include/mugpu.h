@@ -233,16 +233,21 @@ int mugpu_cg_solve_mixed(mugpu_cell cell, const void *d_rhs, void *d_x,
                          double *rel_residual);
 
 /* ---- instrumentation (no reference counterpart) ------------------------
- * HIP-event timing of one kernel class on the library stream, used by the
- * benchmark to measure a kernel's average duration inside a timed run. */
+ * HIP-event timing of one kernel class (or of all, MUGPU_K_ALL) on the
+ * library stream, used by the benchmark to measure kernel durations inside
+ * a timed run.  mugpu_timing_stop returns the launches and milliseconds of
+ * the selected class(es); mugpu_timing_class then reports one class of the
+ * run that was just stopped. */
 #define MUGPU_K_LAST_R2C 0  /* last-axis r2c pass (with the K:p stage)      */
 #define MUGPU_K_MIDDLE 1    /* middle-axis c2c passes (3D)                  */
 #define MUGPU_K_AXIS0 2     /* first-axis pass with the projection          */
 #define MUGPU_K_LAST_C2R 3  /* last-axis c2r pass (with the p.Ap stage)     */
 #define MUGPU_K_MATERIAL 4  /* constitutive kernels                         */
 #define MUGPU_K_CG_UPDATE 5 /* x, r update with the r.r partial sums        */
+#define MUGPU_K_ALL 6       /* every class above                             */
 int mugpu_timing_start(int kernel_class);
 int mugpu_timing_stop(int64_t *launches, double *total_ms);
+int mugpu_timing_class(int kernel_class, int64_t *launches, double *total_ms);
 /* streaming device copy of `bytes` (multiple of 16), the bandwidth
  * reference for the roofline and for the HBM counter calibration */
 int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);

@@ -44,19 +44,27 @@ int64_t g_timing_count = 0;
 constexpr int64_t kMaxTimed = 16384;
 }  // namespace
 
+std::vector<int> g_ev_class;
+int64_t g_stat_launches[MUGPU_K_ALL] = {0};
+double g_stat_ms[MUGPU_K_ALL] = {0};
+
 void timing_begin(int kclass) {
-  if (kclass != g_timing_class || g_timing_count >= kMaxTimed) return;
+  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed)
+    return;
   if ((int64_t)g_ev_start.size() <= g_timing_count) {
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
     g_ev_start.push_back(a);
     g_ev_stop.push_back(b);
+    g_ev_class.push_back(-1);
   }
+  g_ev_class[g_timing_count] = kclass;
   (void)hipEventRecord(g_ev_start[g_timing_count], g_stream);
 }
 
 void timing_end(int kclass) {
-  if (kclass != g_timing_class || g_timing_count >= kMaxTimed) return;
+  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed)
+    return;
   if ((int64_t)g_ev_stop.size() <= g_timing_count) return;
   (void)hipEventRecord(g_ev_stop[g_timing_count], g_stream);
   ++g_timing_count;
@@ -293,15 +301,34 @@ extern "C" int mugpu_timing_stop(int64_t* launches, double* total_ms) {
   MUGPU_TRY(ensure_init());
   MUGPU_HIP(hipStreamSynchronize(g_stream));
   double tot = 0.0;
+  for (int k = 0; k < MUGPU_K_ALL; ++k) {
+    g_stat_launches[k] = 0;
+    g_stat_ms[k] = 0.0;
+  }
   for (int64_t i = 0; i < g_timing_count; ++i) {
     float ms = 0.f;
     MUGPU_HIP(hipEventElapsedTime(&ms, g_ev_start[i], g_ev_stop[i]));
     tot += ms;
+    const int k = g_ev_class[i];
+    if (k >= 0 && k < MUGPU_K_ALL) {
+      ++g_stat_launches[k];
+      g_stat_ms[k] += ms;
+    }
   }
   if (launches) *launches = g_timing_count;
   if (total_ms) *total_ms = tot;
   g_timing_class = -1;
   g_timing_count = 0;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_timing_class(int kernel_class, int64_t* launches, double* total_ms) {
+  if (kernel_class < 0 || kernel_class >= MUGPU_K_ALL) {
+    set_error("kernel class %d out of range", kernel_class);
+    return MUGPU_EARG;
+  }
+  if (launches) *launches = g_stat_launches[kernel_class];
+  if (total_ms) *total_ms = g_stat_ms[kernel_class];
   return MUGPU_OK;
 }
 

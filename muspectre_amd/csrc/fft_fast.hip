@@ -126,6 +126,18 @@ __device__ __forceinline__ c_t<T> czero() {
   return z;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on
+// XCD b mod 8), each XCD with its own L2.  Tiles next to each other in HBM
+// share cache lines at their edges, so the tile of workgroup b is chosen to
+// give every XCD one contiguous range of tiles: neighbours then meet in the
+// same L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
+  const int64_t q = total / kXcds, r = total - q * kXcds;
+  const int64_t x = b % kXcds, local = b / kXcds;
+  return x * q + (x < r ? x : r) + local;
+}
+
 // ---- shapes of the workgroups -----------------------------------------------
 // last-axis passes: rows per workgroup (even: two rows share a complex line)
 __host__ __device__ constexpr int z_rows(int n, int dim) {
@@ -151,9 +163,12 @@ __global__ __launch_bounds__(s_lines<T>() * (N / kE)) void k_strided_fast(
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
-  const int64_t w = (int64_t)blockIdx.x * L + l;
+  const int64_t gx = gridDim.x;
+  const int64_t t = xcd_tile((int64_t)blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+  const int64_t o = t / gx;
+  const int64_t w = (t - o * gx) * L + l;
   const bool valid = w < inner;
-  c_t<T>* base = data + (int64_t)blockIdx.y * N * inner + w;
+  c_t<T>* base = data + o * N * inner + w;
   c_t<T> v[kE];
   load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
   Passes<T, N, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
@@ -180,7 +195,7 @@ __global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * (N / kE)) void k_axis0
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
-  const int64_t q0 = (int64_t)blockIdx.x * KP;
+  const int64_t q0 = xcd_tile(blockIdx.x, gridDim.x) * KP;
   const int64_t inner = a.plane * NC;
   const int64_t w = q0 * NC + l;
   const bool valid = w < inner;
@@ -325,9 +340,23 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int64_t row0 = (int64_t)blockIdx.x * RB;
   {
+    // loads issued in batches of kBatch before their LDS stores, so that each
+    // thread keeps several HBM requests in flight
+    constexpr int TOT = RB * NH * NC, kBatch = 8;
     const int64_t base = row0 * NH * NC, lim = a.rows * NH * NC;
-    for (int e = threadIdx.x; e < RB * NH * NC; e += NT)
-      s[e] = base + e < lim ? a.spec[base + e] : czero<T>();
+    for (int e0 = 0; e0 < TOT; e0 += kBatch * NT) {
+      c_t<T> t[kBatch];
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int e = e0 + i * NT + threadIdx.x;
+        t[i] = (e < TOT && base + e < lim) ? a.spec[base + e] : czero<T>();
+      }
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int e = e0 + i * NT + threadIdx.x;
+        if (e < TOT) s[e] = t[i];
+      }
+    }
   }
   __syncthreads();
   const int l = threadIdx.x % L, p = threadIdx.x / L;
@@ -358,13 +387,29 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c
   const T* sr = reinterpret_cast<const T*>(smem);
   const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
   double acc = 0.0;
-  for (int e = threadIdx.x; e < RB * N * NC; e += NT) {
-    if (base + e >= lim) break;
-    const int r = e / (N * NC), rem = e - r * (N * NC);
-    const int m = rem / NC, c = rem - m * NC;
-    const T val = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
-    a.out[base + e] = val;
-    if (OUT == OUT_DOT) acc += (double)val * (double)a.dotv[base + e];
+  {
+    constexpr int TOT = RB * N * NC, kBatch = 8;
+    for (int e0 = 0; e0 < TOT; e0 += kBatch * NT) {
+      T dv[kBatch];
+      if (OUT == OUT_DOT) {
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) {
+          const int e = e0 + i * NT + threadIdx.x;
+          dv[i] = (e < TOT && base + e < lim) ? a.dotv[base + e] : T(0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int e = e0 + i * NT + threadIdx.x;
+        if (e < TOT && base + e < lim) {
+          const int r = e / (N * NC), rem = e - r * (N * NC);
+          const int m = rem / NC, c = rem - m * NC;
+          const T val = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
+          a.out[base + e] = val;
+          if (OUT == OUT_DOT) acc += (double)val * (double)dv[i];
+        }
+      }
+    }
   }
   if (OUT == OUT_DOT) {
     __syncthreads();
