@@ -346,10 +346,19 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c
     const int64_t base = row0 * NH * NC, lim = a.rows * NH * NC;
     for (int e0 = 0; e0 < TOT; e0 += kBatch * NT) {
       c_t<T> t[kBatch];
+      // clamped, unconditional loads (a predicated load per element would be
+      // compiled as a branch that waits for it); out-of-range values are
+      // replaced afterwards
 #pragma unroll
       for (int i = 0; i < kBatch; ++i) {
         const int e = e0 + i * NT + threadIdx.x;
-        t[i] = (e < TOT && base + e < lim) ? a.spec[base + e] : czero<T>();
+        const int64_t g = base + e < lim ? base + e : lim - 1;
+        t[i] = a.spec[g];
+      }
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int e = e0 + i * NT + threadIdx.x;
+        if (!(e < TOT && base + e < lim)) t[i] = czero<T>();
       }
 #pragma unroll
       for (int i = 0; i < kBatch; ++i) {
@@ -395,7 +404,8 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c
 #pragma unroll
         for (int i = 0; i < kBatch; ++i) {
           const int e = e0 + i * NT + threadIdx.x;
-          dv[i] = (e < TOT && base + e < lim) ? a.dotv[base + e] : T(0);
+          const int64_t g = base + e < lim ? base + e : lim - 1;
+          dv[i] = a.dotv[g];
         }
       }
 #pragma unroll
