@@ -1,16 +1,16 @@
-// FFT passes for axis lengths known at compile time (powers of two from 16
-// to 512: the grids of the large workloads).  Same stages and fusions as the
+// FFT passes for axis lengths known at compile time (16, 32, 48, 64, 96,
+// 128, 256, 512: the grids of the large and of the serving workloads).  Same stages and fusions as the
 // runtime-length passes in fft_passes.hip, laid out for CDNA:
 //
 //  * lines across lanes: a workgroup owns L lines (columns of a tile) and
-//    P = n/16 threads per line; thread (l, p) holds 16 complex values of
-//    line l in registers.  Consecutive lanes work on consecutive lines, so
+//    P = n/E threads per line; thread (l, p) holds E complex values of line
+//    l in registers (E = 16, or 12 for the lengths with a factor 3).  Consecutive lanes work on consecutive lines, so
 //    every LDS access of a pass is unit stride across lanes (element m of
 //    line l at s[m * LS + l], LS = L + 1), and on the strided axes every HBM
 //    access is a contiguous run of L values;
-//  * register-resident Stockham passes of radix 16/8/4 with every index a
-//    compile-time shift or mask (no integer division), one LDS exchange
-//    between passes;
+//  * register-resident Stockham passes of radix 16/8/4/3/2 with every index
+//    a compile-time shift or mask (the span before the radix-3 pass is a
+//    power of two; no integer division), one LDS exchange between passes;
 //  * on the strided axes the first pass reads HBM straight into registers
 //    and the last pass writes HBM straight from registers.
 //
@@ -23,38 +23,51 @@
 namespace mugpu {
 
 bool fast_size(int n) {
-  return n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512;
+  return n == 16 || n == 32 || n == 48 || n == 64 || n == 96 || n == 128 || n == 256 || n == 512;
 }
 
 namespace {
 
-constexpr int kE = 16;  // complex values per thread
+// complex values per thread and threads per line
+__host__ __device__ constexpr int fe(int n) { return n % 3 == 0 ? 12 : 16; }
+__host__ __device__ constexpr int fp(int n) { return n / fe(n); }
 
-__host__ __device__ constexpr int fast_npass(int n) { return n == 16 ? 1 : n == 512 ? 3 : 2; }
+// radices of the passes: each divides fe(n), and the span before every pass
+// is a power of two
+__host__ __device__ constexpr int fast_npass(int n) {
+  return n == 16 ? 1 : n == 512 || n == 48 ? 3 : n == 96 ? 4 : 2;
+}
 __host__ __device__ constexpr int fast_radix(int n, int i) {
-  return n == 16 ? 16 : n == 32 ? (i == 0 ? 8 : 4) : n == 64 ? 8 : n == 128 ? (i == 0 ? 16 : 8)
-                                                                         : n == 256 ? 16 : 8;
+  return n == 16    ? 16
+         : n == 32  ? (i == 0 ? 8 : 4)
+         : n == 48  ? (i < 2 ? 4 : 3)
+         : n == 64  ? 8
+         : n == 96  ? (i < 2 ? 4 : i == 2 ? 2 : 3)
+         : n == 128 ? (i == 0 ? 16 : 8)
+         : n == 256 ? 16
+                    : 8;
 }
 
 template <int N, int R>
 __device__ __forceinline__ int in_idx(int p, int b, int t) {
-  return p + (N / kE) * b + t * (N / R);
+  return p + fp(N) * b + t * (N / R);
 }
 template <int N, int R, int NS>
 __device__ __forceinline__ int out_idx(int p, int b, int t) {
-  const int j = p + (N / kE) * b;
+  const int j = p + fp(N) * b;
   return (j / NS) * NS * R + (j & (NS - 1)) + t * NS;
 }
 
-// twiddles and butterflies of pass (R, NS) on this thread's 16 values
+// twiddles and butterflies of pass (R, NS) on this thread's fe(N) values
 template <typename T, int N, int R, int NS, int SG>
 __device__ __forceinline__ void bfly_pass(c_t<T>* v, int p, const c_t<T>* __restrict__ tw) {
-  constexpr int BPT = kE / R;
+  constexpr int BPT = fe(N) / R;
+  static_assert(BPT * R == fe(N) && (NS & (NS - 1)) == 0, "radix plan");
   constexpr int TWS = N / (NS * R);
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     if (NS > 1) {
-      const int k = (p + (N / kE) * b) & (NS - 1);
+      const int k = (p + fp(N) * b) & (NS - 1);
 #pragma unroll
       for (int t = 1; t < R; ++t) {
         c_t<T> w = tw[t * k * TWS];
@@ -79,12 +92,12 @@ struct Passes {
     if constexpr (I + 1 < fast_npass(N)) {
       constexpr int R2 = fast_radix(N, I + 1);
 #pragma unroll
-      for (int b = 0; b < kE / R; ++b)
+      for (int b = 0; b < fe(N) / R; ++b)
 #pragma unroll
         for (int t = 0; t < R; ++t) at(out_idx<N, R, NS>(p, b, t)) = v[b * R + t];
       __syncthreads();
 #pragma unroll
-      for (int b = 0; b < kE / R2; ++b)
+      for (int b = 0; b < fe(N) / R2; ++b)
 #pragma unroll
         for (int t = 0; t < R2; ++t) v[b * R2 + t] = at(in_idx<N, R2>(p, b, t));
       __syncthreads();
@@ -104,7 +117,7 @@ template <typename T, int N, class Ld>
 __device__ __forceinline__ void load_first(c_t<T>* v, int p, Ld ld) {
   constexpr int R0 = fast_radix(N, 0);
 #pragma unroll
-  for (int b = 0; b < kE / R0; ++b)
+  for (int b = 0; b < fe(N) / R0; ++b)
 #pragma unroll
     for (int t = 0; t < R0; ++t) v[b * R0 + t] = ld(in_idx<N, R0>(p, b, t));
 }
@@ -113,7 +126,7 @@ template <typename T, int N, class St>
 __device__ __forceinline__ void store_last(const c_t<T>* v, int p, St st) {
   constexpr int R = Last<N>::R, NS = Last<N>::NS;
 #pragma unroll
-  for (int b = 0; b < kE / R; ++b)
+  for (int b = 0; b < fe(N) / R; ++b)
 #pragma unroll
     for (int t = 0; t < R; ++t) st(out_idx<N, R, NS>(p, b, t), v[b * R + t]);
 }
@@ -139,13 +152,14 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
 }
 
 // ---- shapes of the workgroups -----------------------------------------------
-// last-axis passes: rows per workgroup (even: two rows share a complex line)
+// last-axis passes: rows per workgroup (even: two rows share a complex
+// line); 288 threads in 3D, 256 in 2D
 __host__ __device__ constexpr int z_rows(int n, int dim) {
-  return dim == 3 ? (n >= 512 ? 2 : 1024 / n) : (n >= 512 ? 4 : 2048 / n);
+  return dim == 3 ? 64 / fp(n) : 128 / fp(n);
 }
-// first-axis projection pass: k-points per workgroup
+// first-axis projection pass: k-points per workgroup (same thread counts)
 __host__ __device__ constexpr int x_kpts(int n, int dim) {
-  return dim == 3 ? (n >= 512 ? 1 : 512 / n) : (n >= 512 ? 2 : 1024 / n);
+  return dim == 3 ? 32 / fp(n) : 64 / fp(n);
 }
 // strided passes: lines per workgroup (a run of 256 bytes)
 template <typename T>
@@ -157,7 +171,7 @@ constexpr int s_lines() {
 // data[(o * N + m) * inner + w]: workgroup (w-tile, o) transforms lines
 // w0 .. w0 + L - 1 of plane o
 template <typename T, int N, int SG>
-__global__ __launch_bounds__(s_lines<T>() * (N / kE)) void k_strided_fast(
+__global__ __launch_bounds__(s_lines<T>() * fp(N)) void k_strided_fast(
     c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
   constexpr int L = s_lines<T>(), LS = L + 1;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -169,7 +183,7 @@ __global__ __launch_bounds__(s_lines<T>() * (N / kE)) void k_strided_fast(
   const int64_t w = (t - o * gx) * L + l;
   const bool valid = w < inner;
   c_t<T>* base = data + o * N * inner + w;
-  c_t<T> v[kE];
+  c_t<T> v[fe(N)];
   load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
   Passes<T, N, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
   if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
@@ -190,8 +204,8 @@ struct XArgs {
 };
 
 template <typename T, int N, int DIM>
-__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * (N / kE)) void k_axis0_fast(XArgs<T> a) {
-  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM), L = KP * NC, LS = L + 1, NT = L * (N / kE);
+__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_fast(XArgs<T> a) {
+  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM), L = KP * NC, LS = L + 1, NT = L * fp(N);
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
@@ -201,7 +215,7 @@ __global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * (N / kE)) void k_axis0
   const bool valid = w < inner;
   c_t<T>* base = a.data + w;
   auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
-  c_t<T> v[kE];
+  c_t<T> v[fe(N)];
   load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
   Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
   store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
@@ -256,9 +270,9 @@ struct LArgs {
 };
 
 template <typename T, int N, int DIM, int IN>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r2c_fast(LArgs<T> a) {
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_fast(LArgs<T> a) {
   constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
-  constexpr int NT = L * (N / kE), NH = N / 2 + 1;
+  constexpr int NT = L * fp(N), NH = N / 2 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
@@ -273,7 +287,7 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r
   } else {
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
     for (int e = threadIdx.x; e < RB * N; e += NT) {
-      const int r = e / N, m = e & (N - 1);
+      const int r = e / N, m = e - (e / N) * N;
       const int64_t row = row0 + r;
       T dp[NC];
       if (row < a.rows) {
@@ -301,7 +315,7 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r
   __syncthreads();
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
-  c_t<T> v[kE];
+  c_t<T> v[fe(N)];
   load_first<T, N>(v, p, [&](int m) { return at(m); });
   __syncthreads();
   Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
@@ -314,7 +328,7 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r
     const int k = rem / NC, c = rem - k * NC;
     const int ln = (r >> 1) * NC + c;
     const c_t<T> z = s[k * LS + ln];
-    const c_t<T> zc = s[((N - k) & (N - 1)) * LS + ln];
+    const c_t<T> zc = s[(k == 0 ? 0 : N - k) * LS + ln];
     c_t<T> X;
     if (!(r & 1)) {
       X.x = T(0.5) * (z.x + zc.x);
@@ -333,9 +347,9 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_r
 // Z = A + i B, whose backward transform holds row 2 rp in its real and row
 // 2 rp + 1 in its imaginary part.
 template <typename T, int N, int DIM, int OUT>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c2r_fast(LArgs<T> a) {
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_fast(LArgs<T> a) {
   constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
-  constexpr int NT = L * (N / kE), NH = N / 2 + 1;
+  constexpr int NT = L * fp(N), NH = N / 2 + 1;
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int64_t row0 = (int64_t)blockIdx.x * RB;
@@ -370,7 +384,7 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / kE)) void k_c
   __syncthreads();
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   const int rp = l / NC, cc = l - rp * NC;
-  c_t<T> v[kE];
+  c_t<T> v[fe(N)];
   load_first<T, N>(v, p, [&](int i) {
     const int kk = i <= N / 2 ? i : N - i;
     c_t<T> A = s[((2 * rp) * NH + kk) * NC + cc];
@@ -451,7 +465,7 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.spec = reinterpret_cast<c_t<T>*>(spec);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
-  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / kE);
+  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * fp(N);
   const size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
   const dim3 grid((unsigned)((a.rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)));
   switch (op.in_mode) {
@@ -473,7 +487,7 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
   constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
-  constexpr int NT = RB / 2 * NC * (N / kE);
+  constexpr int NT = RB / 2 * NC * fp(N);
   const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(c_t<T>);
   size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
   if (tile > lds) lds = tile;
@@ -494,7 +508,7 @@ int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   int64_t outer = 1;
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   constexpr int L = s_lines<T>();
-  constexpr int NT = L * (N / kE);
+  constexpr int NT = L * fp(N);
   const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
   c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
@@ -520,7 +534,7 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   }
   a.hlast = p.hres[DIM - 1];
   a.norm = (T)pj.norm;
-  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * (N / kE);
+  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * fp(N);
   const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
   hipLaunchKernelGGL((k_axis0_fast<T, N, DIM>), grid, dim3(NT), lds, stream(), a);
@@ -533,7 +547,9 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   switch (n) {                     \
     case 16: return CALL(16);      \
     case 32: return CALL(32);      \
+    case 48: return CALL(48);      \
     case 64: return CALL(64);      \
+    case 96: return CALL(96);      \
     case 128: return CALL(128);    \
     case 256: return CALL(256);    \
     case 512: return CALL(512);    \

@@ -20,7 +20,8 @@ SIZES = [[6, 4], [4, 6], [5, 5], [3, 3], [51, 51], [17, 4], [3, 5, 7], [7, 7, 7]
          [16, 16, 16], [8, 6, 4], [9, 11, 13], [32, 32, 32], [12, 10, 8],
          [125, 5, 3], [2, 2, 2], [1, 10], [64, 48, 40], [256, 128], [16, 512],
          [128, 32, 64], [16, 256, 32], [32, 9, 16], [512, 3, 16], [96, 16, 64],
-         [3, 512], [64, 64], [7, 128, 5]]
+         [3, 512], [64, 64], [7, 128, 5], [48, 48, 48], [96, 32, 48], [48, 96],
+         [96, 96, 96], [5, 48, 96]]
 
 
 def _fft_plan(capi, res, prec):
@@ -125,7 +126,7 @@ def test_projection_rejects_even_grid(capi):
 
 
 @pytest.mark.parametrize("res", [[8, 8], [16, 12, 10], [64, 64, 64], [32, 64], [512, 16],
-                                 [128, 16, 32], [16, 5, 256]],
+                                 [128, 16, 32], [16, 5, 256], [48, 96, 48], [96, 48]],
                          ids=lambda r: "x".join(map(str, r)))
 @pytest.mark.parametrize("kind", [0, 1])
 def test_projection_even_policy_vs_oracle(capi, oracle, res, kind):
