@@ -232,6 +232,70 @@ int mugpu_cg_solve_mixed(mugpu_cell cell, const void *d_rhs, void *d_x,
                          int64_t *iterations, int *refinements,
                          double *rel_residual);
 
+/* ---- serving loop (no reference counterpart) ----------------------------
+ * Many independent RVE solves of uneven resolution on one GPU.  Each request
+ * is one small-strain load step of newton_cg (src/solver/solvers.cc:44-217)
+ * on a cell with per-voxel isotropic linear elasticity (Young's modulus and
+ * Poisson ratio per voxel, MaterialLinearElastic4, material_linear_elastic4
+ * .hh:189-210), solved with the mixed-precision SolverCG (FP32 inner CG,
+ * FP64 refinement).  Requests are admitted first come, first served while
+ * their device footprint fits the HBM budget; the inner CG iterations of all
+ * running requests of one resolution run in shared, graph-captured batched
+ * launches; a request's Newton and refinement steps run between them on a
+ * second stream.  Each request computes exactly what it computes alone.
+ * Resolutions: every axis in {16, 32, 48, 64, 96, 128, 256, 512}; even
+ * lengths use the Nyquist policy (allow_even). */
+typedef struct mugpu_server_s *mugpu_server;
+
+typedef struct {
+  size_t hbm_budget;    /* device bytes for admitted requests; 0: 3/4 of free */
+  int max_batch;        /* slots per resolution (1..64) */
+  int iters_per_launch; /* inner CG iterations per batched launch (>= 1) */
+  int use_graphs;       /* capture the batched iterations in hipGraphs */
+  int keep_fields;      /* keep strain and stress fields of finished requests */
+  double cg_tol;        /* SolverCG tolerance, relative to |rhs| */
+  int cg_maxiter;       /* SolverCG maxiter (also bounds the Newton loop) */
+  double inner_tol;     /* FP32 inner CG tolerance */
+  double newton_tol;
+  double equil_tol;
+} mugpu_server_config;
+
+int mugpu_server_create(const mugpu_server_config *config, mugpu_server *out);
+int mugpu_server_destroy(mugpu_server server);
+/* device bytes one request of this resolution occupies while it runs */
+int mugpu_server_footprint(int dim, const int *resolutions, size_t *bytes);
+/* queue one solve; young/poisson per voxel (row-major pixels), strain the
+ * symmetric dim x dim macroscopic strain (column-major); cg_maxiter 0 takes
+ * the server's.  MUGPU_ESOLVER for an asymmetric strain, MUGPU_EMEMORY when
+ * the footprint exceeds the whole budget, MUGPU_EARG for a resolution
+ * without compiled passes. */
+int mugpu_server_submit(mugpu_server server, int dim, const int *resolutions,
+                        const double *lengths, const double *young,
+                        const double *poisson, const double *strain,
+                        int cg_maxiter, int64_t *id);
+/* one turn of the loop: admit, advance every request and batch whose device
+ * work has completed, launch what is ready; with wait != 0 and nothing
+ * completed, block until some device work completes.  *unfinished counts
+ * queued and running requests. */
+int mugpu_server_progress(mugpu_server server, int wait, int *unfinished);
+/* ids of the requests finished since the last call (at most capacity) */
+int mugpu_server_poll(mugpu_server server, int64_t *ids, int capacity,
+                      int *count);
+/* result of a finished request: status (MUGPU_OK, or MUGPU_ECONVERGENCE when
+ * the CG failed), whether newton_cg converged, Newton and CG iteration
+ * counts, the homogenised stress (dim*dim, column-major) and, when the
+ * server keeps fields and the pointers are not NULL, the strain and stress
+ * fields (nb_pixels x dim*dim, AoS) */
+int mugpu_server_result(mugpu_server server, int64_t id, int *status,
+                        int *converged, int *newton_iterations,
+                        int64_t *cg_iterations, double *mean_stress,
+                        double *strain, double *stress);
+int mugpu_server_release(mugpu_server server, int64_t id);
+/* counters: [0] submitted, [1] admitted, [2] finished, [3] failed,
+ * [4] batched launches, [5] summed slots over batched launches,
+ * [6] peak device bytes in use, [7] budget, [8] peak requests running */
+int mugpu_server_stats(mugpu_server server, int64_t *out, int n);
+
 /* ---- instrumentation (no reference counterpart) ------------------------
  * HIP-event timing of one kernel class (or of all, MUGPU_K_ALL) on the
  * library stream, used by the benchmark to measure kernel durations inside

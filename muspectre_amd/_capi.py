@@ -16,6 +16,15 @@ FINITE_STRAIN, SMALL_STRAIN = 0, 1
 PROJ_SMALL_STRAIN, PROJ_FINITE_STRAIN_FAST, PROJ_FINITE_STRAIN = 0, 1, 2
 LE1, LE2, LE3, LE4, HEP1, HEP1_CT = 1, 2, 3, 4, 5, 6
 
+class ServerConfig(ctypes.Structure):
+    """mugpu_server_config of include/mugpu.h."""
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_batch", ctypes.c_int),
+                ("iters_per_launch", ctypes.c_int), ("use_graphs", ctypes.c_int),
+                ("keep_fields", ctypes.c_int), ("cg_tol", ctypes.c_double),
+                ("cg_maxiter", ctypes.c_int), ("inner_tol", ctypes.c_double),
+                ("newton_tol", ctypes.c_double), ("equil_tol", ctypes.c_double)]
+
+
 # name, restype, argtypes
 _vp = ctypes.c_void_p
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -76,6 +85,15 @@ SIGNATURES = [
     ("mugpu_timing_start", _int, [_int]),
     ("mugpu_timing_stop", _int, [_i64p, _dp]),
     ("mugpu_timing_class", _int, [_int, _i64p, _dp]),
+    ("mugpu_server_create", _int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
+    ("mugpu_server_destroy", _int, [_vp]),
+    ("mugpu_server_footprint", _int, [_int, _ip, ctypes.POINTER(ctypes.c_size_t)]),
+    ("mugpu_server_submit", _int, [_vp, _int, _ip, _dp, _dp, _dp, _dp, _int, _i64p]),
+    ("mugpu_server_progress", _int, [_vp, _int, _ip]),
+    ("mugpu_server_poll", _int, [_vp, _i64p, _int, _ip]),
+    ("mugpu_server_result", _int, [_vp, _i64, _ip, _ip, _ip, _i64p, _dp, _dp, _dp]),
+    ("mugpu_server_release", _int, [_vp, _i64]),
+    ("mugpu_server_stats", _int, [_vp, _i64p, _int]),
     ("mugpu_copy", _int, [_vp, _vp, ctypes.c_size_t]),
 ]
 K_LAST_R2C, K_MIDDLE, K_AXIS0, K_LAST_C2R, K_MATERIAL, K_CG_UPDATE = range(6)

@@ -34,7 +34,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
   return MUGPU_EHIP;
 }
 
-hipStream_t stream() { return g_stream; }
+hipStream_t g_cur_stream = nullptr;
+hipStream_t stream() { return g_cur_stream ? g_cur_stream : g_stream; }
+void set_current_stream(hipStream_t s) { g_cur_stream = s; }
+bool g_capturing = false;
+void set_capturing(bool on) { g_capturing = on; }
 int max_lds_per_block() { return g_maxlds; }
 
 namespace {
@@ -49,7 +53,8 @@ int64_t g_stat_launches[MUGPU_K_ALL] = {0};
 double g_stat_ms[MUGPU_K_ALL] = {0};
 
 void timing_begin(int kclass) {
-  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed)
+  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed ||
+      g_capturing)
     return;
   if ((int64_t)g_ev_start.size() <= g_timing_count) {
     hipEvent_t a, b;
@@ -59,14 +64,15 @@ void timing_begin(int kclass) {
     g_ev_class.push_back(-1);
   }
   g_ev_class[g_timing_count] = kclass;
-  (void)hipEventRecord(g_ev_start[g_timing_count], g_stream);
+  (void)hipEventRecord(g_ev_start[g_timing_count], stream());
 }
 
 void timing_end(int kclass) {
-  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed)
+  if ((kclass != g_timing_class && g_timing_class != MUGPU_K_ALL) || g_timing_count >= kMaxTimed ||
+      g_capturing)
     return;
   if ((int64_t)g_ev_stop.size() <= g_timing_count) return;
-  (void)hipEventRecord(g_ev_stop[g_timing_count], g_stream);
+  (void)hipEventRecord(g_ev_stop[g_timing_count], stream());
   ++g_timing_count;
 }
 

@@ -145,7 +145,89 @@ __global__ __launch_bounds__(kRT) void k_copy(const double2* __restrict__ s,
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) d[i] = s[i];
 }
 
+// ---- batched inner CG: one RVE per blockIdx (finalize) or blockIdx.z ----
+// same arithmetic and reduction order as k_finalize / k_cg_update, so a slot
+// computes exactly what the unbatched FP32 inner CG computes
+__global__ __launch_bounds__(1024) void k_finalize_batch(const BatchSlot* __restrict__ slots,
+                                                         int64_t count, int mode) {
+  const BatchSlot& sl = slots[blockIdx.x];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  __shared__ double red[1024];
+  const double* partials = mode == FIN_ALPHA ? sl.zpart : sl.upart;
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < count; i += 1024) acc += partials[i];
+  const double s = block_reduce<1024>(acc, red);
+  if (threadIdx.x == 0) {
+    double* scal = sl.scal;
+    if (mode == FIN_ALPHA) {
+      scal[SC_PAP] = s;
+      scal[SC_ALPHA] = scal[SC_RDR] / s;
+    } else {
+      scal[SC_NEWRDR] = s;
+      scal[SC_BETA] = s / scal[SC_RDR];
+      scal[SC_RDR] = s;
+      // loop test of the inner CG: iterations left and r.r above the
+      // threshold (a NaN fails the comparison and stops the solve too)
+      const int it = sl.ctl[CTL_ITER] + 1;
+      sl.ctl[CTL_ITER] = it;
+      if (!(it < sl.ctl[CTL_MAXIT] && s > scal[SC_TOL2])) sl.ctl[CTL_ACTIVE] = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kRT) void k_cg_update_batch(const BatchSlot* __restrict__ slots,
+                                                         int64_t n) {
+  const BatchSlot& sl = slots[blockIdx.z];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  __shared__ double red[kRT];
+  float* __restrict__ x = sl.x;
+  float* __restrict__ r = sl.r;
+  const float* __restrict__ p = sl.p;
+  const float* __restrict__ Ap = sl.Ap;
+  const double alpha = sl.scal[SC_ALPHA];
+  const float al = (float)alpha;
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    x[i] += al * p[i];
+    const float rv = r[i] + al * Ap[i];
+    r[i] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  const double t = block_reduce<kRT>(acc, red);
+  if (threadIdx.x == 0) sl.upart[blockIdx.x] = t;
+}
+
 }  // namespace
+
+int batch_finalize(const BatchSlot* d_slots, int nslots, int64_t count, int mode) {
+  if (nslots <= 0) return MUGPU_OK;
+  hipLaunchKernelGGL(k_finalize_batch, dim3((unsigned)nslots), dim3(1024), 0, stream(), d_slots,
+                     count, mode);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int batch_cg_update(const BatchSlot* d_slots, int nslots, int64_t n) {
+  if (nslots <= 0) return MUGPU_OK;
+  const unsigned g = grid_for(n);
+  timing_begin(MUGPU_K_CG_UPDATE);
+  hipLaunchKernelGGL(k_cg_update_batch, dim3(g, 1, (unsigned)nslots), dim3(kRT), 0, stream(),
+                     d_slots, n);
+  timing_end(MUGPU_K_CG_UPDATE);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots) {
+  const int64_t n = p.N * p.nc;
+  MUGPU_TRY(fast_batch_operator(p, d_slots, nslots));
+  MUGPU_TRY(batch_finalize(d_slots, nslots, p.nzblocks, FIN_ALPHA));
+  MUGPU_TRY(batch_cg_update(d_slots, nslots, n));
+  return batch_finalize(d_slots, nslots, grid_for(n), FIN_BETA);
+}
+
+int reduction_blocks(int64_t n) { return (int)grid_for(n); }
 
 int copy_device(void* dst, const void* src, size_t bytes) {
   const int64_t n = (int64_t)(bytes / 16);

@@ -171,14 +171,13 @@ constexpr int s_lines() {
 // data[(o * N + m) * inner + w]: workgroup (w-tile, o) transforms lines
 // w0 .. w0 + L - 1 of plane o
 template <typename T, int N, int SG>
-__global__ __launch_bounds__(s_lines<T>() * fp(N)) void k_strided_fast(
-    c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
+__device__ __forceinline__ void strided_body(c_t<T>* __restrict__ data, int64_t inner,
+                                             const c_t<T>* __restrict__ tw, unsigned char* smem,
+                                             int64_t bx, int64_t by, int64_t gx, int64_t gy) {
   constexpr int L = s_lines<T>(), LS = L + 1;
-  extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
-  const int64_t gx = gridDim.x;
-  const int64_t t = xcd_tile((int64_t)blockIdx.y * gx + blockIdx.x, gx * gridDim.y);
+  const int64_t t = xcd_tile(by * gx + bx, gx * gy);
   const int64_t o = t / gx;
   const int64_t w = (t - o * gx) * L + l;
   const bool valid = w < inner;
@@ -187,6 +186,24 @@ __global__ __launch_bounds__(s_lines<T>() * fp(N)) void k_strided_fast(
   load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
   Passes<T, N, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
   if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
+}
+
+template <typename T, int N, int SG>
+__global__ __launch_bounds__(s_lines<T>() * fp(N)) void k_strided_fast(
+    c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  strided_body<T, N, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
+// batched (FP32, one RVE per blockIdx.z)
+template <int N, int SG>
+__global__ __launch_bounds__(s_lines<float>() * fp(N)) void k_strided_batch(
+    const BatchSlot* __restrict__ slots, int64_t inner, const float2* __restrict__ tw) {
+  const BatchSlot& sl = slots[blockIdx.z];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  extern __shared__ __align__(16) unsigned char smem[];
+  strided_body<float, N, SG>(sl.spec, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x,
+                             gridDim.y);
 }
 
 // ---- first axis: forward c2c, projection, backward c2c --------------------
@@ -204,12 +221,12 @@ struct XArgs {
 };
 
 template <typename T, int N, int DIM>
-__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_fast(XArgs<T> a) {
+__device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* smem, int64_t bx,
+                                           int64_t gx) {
   constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM), L = KP * NC, LS = L + 1, NT = L * fp(N);
-  extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
-  const int64_t q0 = xcd_tile(blockIdx.x, gridDim.x) * KP;
+  const int64_t q0 = xcd_tile(bx, gx) * KP;
   const int64_t inner = a.plane * NC;
   const int64_t w = q0 * NC + l;
   const bool valid = w < inner;
@@ -249,6 +266,27 @@ __global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_fa
   if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
 }
 
+template <typename T, int N, int DIM>
+__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_fast(XArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  axis0_body<T, N, DIM>(a, smem, blockIdx.x, gridDim.x);
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_batch(
+    const BatchSlot* __restrict__ slots, XArgs<float> shape) {
+  const BatchSlot& sl = slots[blockIdx.z];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  XArgs<float> a = shape;
+  a.data = sl.spec;
+  a.kind = sl.kind;
+  a.allow_even = sl.allow_even;
+  for (int d = 0; d < 3; ++d) a.L[d] = sl.L[d];
+  a.norm = (float)sl.norm;
+  extern __shared__ __align__(16) unsigned char smem[];
+  axis0_body<float, N, DIM>(a, smem, blockIdx.x, gridDim.x);
+}
+
 // ---- last axis: input stage, r2c ---------------------------------------------
 // Rows 2 rp and 2 rp + 1 of component c share complex line rp * NC + c (row
 // 2 rp in the real part), separated after the transform by Hermitian symmetry:
@@ -270,13 +308,12 @@ struct LArgs {
 };
 
 template <typename T, int N, int DIM, int IN>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_fast(LArgs<T> a) {
+__device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
   constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
   constexpr int NT = L * fp(N), NH = N / 2 + 1;
-  extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
-  const int64_t row0 = (int64_t)blockIdx.x * RB;
+  const int64_t row0 = bx * RB;
   if (IN == IN_PLAIN) {
     const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
     for (int e = threadIdx.x; e < RB * N * NC; e += NT) {
@@ -341,18 +378,45 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_
   }
 }
 
+template <typename T, int N, int DIM, int IN>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_fast(LArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  r2c_body<T, N, DIM, IN>(a, smem, blockIdx.x);
+}
+
+// batched: the first iteration of a solve takes p as it is, later ones
+// update it (p = -r + beta p) on the way in
+template <int N, int DIM>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_batch(
+    const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
+  const BatchSlot& sl = slots[blockIdx.z];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  LArgs<float> a{};
+  a.in = sl.p;
+  a.rvec = sl.r;
+  a.pout = sl.p;
+  a.scal = sl.scal;
+  a.tan = sl.tan;
+  a.tan_kind = sl.tan_kind;
+  a.spec = sl.spec;
+  a.rows = rows;
+  a.tw = tw;
+  extern __shared__ __align__(16) unsigned char smem[];
+  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, DIM, IN_TAN>(a, smem, blockIdx.x);
+  else r2c_body<float, N, DIM, IN_TAN_PUPDATE>(a, smem, blockIdx.x);
+}
+
 // ---- last axis: c2r, output stage ---------------------------------------------
 // Half lines A (row 2 rp) and B (row 2 rp + 1) are completed by Hermitian
 // symmetry (imaginary parts of k = 0 and k = n/2 discarded) and combined into
 // Z = A + i B, whose backward transform holds row 2 rp in its real and row
 // 2 rp + 1 in its imaginary part.
 template <typename T, int N, int DIM, int OUT>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_fast(LArgs<T> a) {
+__device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
   constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
   constexpr int NT = L * fp(N), NH = N / 2 + 1;
-  extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
-  const int64_t row0 = (int64_t)blockIdx.x * RB;
+  const int64_t row0 = bx * RB;
   {
     // loads issued in batches of kBatch before their LDS stores, so that each
     // thread keeps several HBM requests in flight
@@ -438,8 +502,30 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_
   if (OUT == OUT_DOT) {
     __syncthreads();
     const double t = block_sum_any<NT>(acc, reinterpret_cast<double*>(smem));
-    if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+    if (threadIdx.x == 0) a.partials[bx] = t;
   }
+}
+
+template <typename T, int N, int DIM, int OUT>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_fast(LArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  c2r_body<T, N, DIM, OUT>(a, smem, blockIdx.x);
+}
+
+template <int N, int DIM>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_batch(
+    const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
+  const BatchSlot& sl = slots[blockIdx.z];
+  if (!sl.ctl[CTL_ACTIVE]) return;
+  LArgs<float> a{};
+  a.spec = sl.spec;
+  a.out = sl.Ap;
+  a.dotv = sl.p;
+  a.partials = sl.zpart;
+  a.rows = rows;
+  a.tw = tw;
+  extern __shared__ __align__(16) unsigned char smem[];
+  c2r_body<float, N, DIM, OUT_DOT>(a, smem, blockIdx.x);
 }
 
 // ---- launchers ----------------------------------------------------------------
@@ -585,9 +671,124 @@ int axis0_any(FftPlan& p, const ProjParams& pj, void* spec) {
   return MUGPU_EARG;
 }
 
+// ---- batched launchers (FP32, slots on blockIdx.z) ------------------------------
+template <int N, int DIM>
+int b_r2c(FftPlan& p, const BatchSlot* slots, int ns) {
+  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * fp(N);
+  const int64_t rows = p.N / N;
+  const size_t lds = lines_lds<N, DIM>(sizeof(float2));
+  const dim3 grid((unsigned)((rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)), 1, (unsigned)ns);
+  hipLaunchKernelGGL((k_r2c_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+                     tw_of<float>(p, DIM - 1));
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <int N, int DIM>
+int b_c2r(FftPlan& p, const BatchSlot* slots, int ns) {
+  constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
+  constexpr int NT = RB / 2 * NC * fp(N);
+  const int64_t rows = p.N / N;
+  const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(float2);
+  size_t lds = lines_lds<N, DIM>(sizeof(float2));
+  if (tile > lds) lds = tile;
+  if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
+  const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
+  hipLaunchKernelGGL((k_c2r_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+                     tw_of<float>(p, DIM - 1));
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <int N>
+int b_strided(FftPlan& p, int ax, int sign, const BatchSlot* slots, int ns) {
+  int64_t inner = p.nc;
+  for (int d = ax + 1; d < p.dim; ++d) inner *= p.hres[d];
+  int64_t outer = 1;
+  for (int d = 0; d < ax; ++d) outer *= p.hres[d];
+  constexpr int L = s_lines<float>();
+  constexpr int NT = L * fp(N);
+  const size_t lds = (size_t)N * (L + 1) * sizeof(float2);
+  const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer, (unsigned)ns);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_strided_batch<N, -1>), grid, dim3(NT), lds, stream(), slots, inner,
+                       tw_of<float>(p, ax));
+  else
+    hipLaunchKernelGGL((k_strided_batch<N, +1>), grid, dim3(NT), lds, stream(), slots, inner,
+                       tw_of<float>(p, ax));
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <int N, int DIM>
+int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
+  XArgs<float> a{};
+  a.plane = p.kpts / p.res[0];
+  a.tw = tw_of<float>(p, 0);
+  for (int d = 0; d < 3; ++d) a.res[d] = p.res[d];
+  a.hlast = p.hres[DIM - 1];
+  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * fp(N);
+  const size_t lds = (size_t)N * (L + 1) * sizeof(float2);
+  const dim3 grid((unsigned)((a.plane + KP - 1) / KP), 1, (unsigned)ns);
+  hipLaunchKernelGGL((k_axis0_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, a);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+template <int DIM>
+int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
+  int rc = MUGPU_EARG;
+  timing_begin(MUGPU_K_LAST_R2C);
+#define C_(n) b_r2c<n, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], C_) return MUGPU_EARG; }();
+#undef C_
+  timing_end(MUGPU_K_LAST_R2C);
+  if (rc) return rc;
+  if (DIM == 3) {
+    timing_begin(MUGPU_K_MIDDLE);
+#define C_(n) b_strided<n>(p, 1, -1, slots, ns)
+    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], C_) return MUGPU_EARG; }();
+#undef C_
+    timing_end(MUGPU_K_MIDDLE);
+    if (rc) return rc;
+  }
+  timing_begin(MUGPU_K_AXIS0);
+#define C_(n) b_axis0<n, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], C_) return MUGPU_EARG; }();
+#undef C_
+  timing_end(MUGPU_K_AXIS0);
+  if (rc) return rc;
+  if (DIM == 3) {
+    timing_begin(MUGPU_K_MIDDLE);
+#define C_(n) b_strided<n>(p, 1, +1, slots, ns)
+    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], C_) return MUGPU_EARG; }();
+#undef C_
+    timing_end(MUGPU_K_MIDDLE);
+    if (rc) return rc;
+  }
+  timing_begin(MUGPU_K_LAST_C2R);
+#define C_(n) b_c2r<n, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], C_) return MUGPU_EARG; }();
+#undef C_
+  timing_end(MUGPU_K_LAST_C2R);
+  return rc;
+}
+
 }  // namespace
 
 int fast_rows_per_block(int n, int dim) { return z_rows(n, dim); }
+
+int fast_batch_operator(FftPlan& p, const BatchSlot* d_slots, int nslots) {
+  for (int d = 0; d < p.dim; ++d)
+    if (!p.fastax[d]) {
+      set_error("batched solves need axis lengths with compiled passes (16, 32, 48, 64, 96, 128, 256, 512), got %d",
+                p.res[d]);
+      return MUGPU_EARG;
+    }
+  if (nslots <= 0) return MUGPU_OK;
+  return p.dim == 3 ? batch_operator_dim<3>(p, d_slots, nslots)
+                    : batch_operator_dim<2>(p, d_slots, nslots);
+}
 
 int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec) {
   if (prec == MUGPU_FP64) return p.dim == 3 ? r2c_any<double, 3>(p, op, spec) : r2c_any<double, 2>(p, op, spec);

@@ -600,7 +600,7 @@ int launch_le(const MaterialDev& m, MatArgs& a, dim3 grid) {
 
 int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,
                   int tan_kind, double* tangent, int need_tangent) {
-  const int64_t n = (int64_t)m.h_pix.size();
+  const int64_t n = m.n_pixels >= 0 ? m.n_pixels : (int64_t)m.h_pix.size();
   if (n == 0) return MUGPU_OK;
   MatArgs a{};
   a.pix = m.d_pix;

@@ -27,6 +27,11 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
   } while (0)
 
 hipStream_t stream();
+// route the library's launches to another stream (nullptr: the library
+// stream); used by the serving loop, which drives two streams from one thread
+void set_current_stream(hipStream_t s);
+// while a stream is captured into a graph, the per-launch event timing is off
+void set_capturing(bool on);
 int max_lds_per_block();
 // per-launch event timing of one kernel class (MUGPU_K_*), off by default
 void timing_begin(int kclass);
@@ -69,7 +74,8 @@ __host__ __device__ constexpr int sym_index(int r, int c, int n) {
 }
 
 // device scalar block used by the CG kernels
-enum Scalar { SC_RDR = 0, SC_PAP, SC_ALPHA, SC_NEWRDR, SC_BETA, SC_TMP0, SC_TMP1, SC_COUNT };
+// SC_TOL2: squared stopping threshold of a batched inner CG
+enum Scalar { SC_RDR = 0, SC_PAP, SC_ALPHA, SC_NEWRDR, SC_BETA, SC_TMP0, SC_TMP1, SC_TOL2, SC_COUNT };
 
 // ---- FFT plan -----------------------------------------------------------------
 struct FftPlan {
@@ -127,13 +133,48 @@ int run_operator(FftPlan& p, int prec, const OpSpec& op, const ProjParams& pj,
 int run_r2c(FftPlan& p, int prec, const void* in, void* spectrum);
 int run_c2r(FftPlan& p, int prec, void* spectrum, void* out);
 
-// passes for axis lengths 16, 32, ..., 512 (fft_fast.hip)
+struct BatchSlot;
+// passes for axis lengths 16, 32, 48, 64, 96, 128, 256, 512 (fft_fast.hip)
 bool fast_size(int n);
 int fast_rows_per_block(int n, int dim);
 int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_c2r(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_strided(FftPlan& p, int prec, int ax, void* spec, int sign);
 int fast_axis0_project(FftPlan& p, int prec, const ProjParams& pj, void* spec);
+// batched FP32 operator G:(K:p) with the p update and the p . Ap partials,
+// one RVE per slot (fft_fast.hip)
+int fast_batch_operator(FftPlan& p, const BatchSlot* d_slots, int nslots);
+// batched k_finalize (alpha or beta + stopping test) and x, r update (cg.hip)
+int batch_finalize(const BatchSlot* d_slots, int nslots, int64_t count, int mode);
+int batch_cg_update(const BatchSlot* d_slots, int nslots, int64_t n);
+
+// ---- batched inner CG (serving) -------------------------------------------
+// One RVE of a batch of equal-resolution RVEs whose FP32 inner CG iterations
+// run in the same launches (slot = blockIdx.z).  ctl[0] active flag, ctl[1]
+// iterations done, ctl[2] iteration limit; a finished slot is skipped by
+// every later launch, so a solve stops exactly where it would stop alone.
+struct BatchSlot {
+  const float* tan;
+  float* x;
+  float* r;
+  float* p;
+  float* Ap;
+  float2* spec;
+  double* scal;   // SC_* block of this RVE
+  double* zpart;  // per-workgroup partials of the c2r pass (p . Ap)
+  double* upart;  // per-workgroup partials of the update (r . r)
+  int* ctl;
+  int tan_kind;
+  int kind;        // projection (MUGPU_PROJ_*)
+  int allow_even;
+  double L[3];
+  double norm;
+};
+enum BatchCtl { CTL_ACTIVE = 0, CTL_ITER = 1, CTL_MAXIT = 2, CTL_COUNT = 4 };
+
+// one inner CG iteration for the slots [0, nslots) of a batch (FP32 operator,
+// FP64 scalars); slots must share the plan's resolution
+int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots);
 
 // ---- reductions and vector kernels (cg.hip) --------------------------------
 int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,
@@ -149,6 +190,8 @@ int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, d
 int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n);
 int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add);
 constexpr int kRedBlocks = 1024;
+// workgroups (= partial sums) of the reductions over n values
+int reduction_blocks(int64_t n);
 
 // ---- materials (materials.hip) ----------------------------------------------
 struct MaterialDev {
@@ -163,6 +206,7 @@ struct MaterialDev {
   double* be[2] = {nullptr, nullptr};
   double* ep[2] = {nullptr, nullptr};
   int cur = 0;
+  int64_t n_pixels = -1;  // pixel count when h_pix is not kept on the host
 };
 
 int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,

@@ -1,0 +1,153 @@
+"""The serving loop on the GPU.
+
+A request is one small-strain newton_cg load step (src/solver/solvers.cc:
+44-217) with the mixed-precision SolverCG on per-voxel isotropic elasticity
+(MaterialLinearElastic4).  Checks:
+  * a request served alone computes exactly what newton_cg computes on a Cell
+    with the same material and precision (bit-identical fields and counts),
+    which ties the server to the parity-tested cell path;
+  * every request of a mix (several resolutions, more requests than slots,
+    slots changing hands between batched launches) gives exactly its alone
+    result;
+  * a request whose CG fails (ConvergenceError) fails alone and leaves the
+    others untouched;
+  * admission keeps the device footprint within the budget;
+  * replaying captured graphs and launching kernel by kernel agree.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def µ():
+    import muspectre_amd
+    return muspectre_amd
+
+
+@pytest.fixture(scope="module")
+def serving():
+    from muspectre_amd import serving
+    return serving
+
+
+def _grf_phase(res, ell, seed):
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal(res)
+    k2 = sum(np.meshgrid(*[np.fft.fftfreq(n) ** 2 for n in res], indexing="ij"))
+    f = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-0.5 * (2 * np.pi) ** 2 * k2 * ell ** 2)))
+    return f > np.median(f)
+
+
+def _request(res, seed):
+    """C4-style request: thresholded GRF phases, contrast log-uniform in
+    [1, 100], nu in [0.2, 0.4], one symmetric strain with entries in +-0.01."""
+    rng = np.random.default_rng(1000 + seed)
+    mask = _grf_phase(res, max(2.0, res[0] / 16), seed)
+    contrast = np.exp(rng.uniform(0, np.log(100)))
+    E = np.where(mask.ravel(), 10e9 * contrast, 10e9)
+    nu = np.full(mask.size, rng.uniform(0.2, 0.4))
+    d = len(res)
+    a = rng.uniform(-0.01, 0.01, (d, d))
+    return {"res": list(res), "E": E, "nu": nu, "strain": 0.5 * (a + a.T)}
+
+
+def _serve(serving, reqs, **kw):
+    kw.setdefault("hbm_budget", 2 << 30)
+    srv = serving.RVEServer(keep_fields=True, **kw)
+    ids = [srv.submit(q["res"], q["E"], q["nu"], q["strain"], cg_maxiter=q.get("maxiter", 0))
+           for q in reqs]
+    srv.run()
+    done = set(srv.poll())
+    assert done == set(ids)
+    out = [srv.result(i, fields=True) for i in ids]
+    st = srv.stats()
+    srv.close()
+    return out, st
+
+
+def _alone(serving, reqs, **kw):
+    return [_serve(serving, [q], **kw)[0][0] for q in reqs]
+
+
+def _same(a, b):
+    assert a["status"] == b["status"] and a["converged"] == b["converged"]
+    assert a["newton_iterations"] == b["newton_iterations"]
+    assert a["cg_iterations"] == b["cg_iterations"]
+    np.testing.assert_array_equal(a["strain"], b["strain"])
+    np.testing.assert_array_equal(a["stress"], b["stress"])
+    np.testing.assert_array_equal(a["mean_stress"], b["mean_stress"])
+
+
+@pytest.mark.parametrize("res", [[64, 64], [32, 32, 32], [48, 16, 32]], ids=lambda r: "x".join(map(str, r)))
+def test_request_equals_cell_newton_cg(µ, serving, res):
+    q = _request(res, 3)
+    out, _ = _serve(serving, [q])
+    r = out[0]
+    d = len(res)
+    cell = µ.Cell(res, [1.0] * d, µ.Formulation.small_strain, precision="fp32", allow_even=True)
+    mat = getattr(µ.material, "MaterialLinearElastic4_{}d".format(d)).make(cell, "rve")
+    mat.add_pixels_by_index(np.arange(q["E"].size), np.stack([q["E"], q["nu"]], axis=1))
+    solver = µ.solvers.SolverCG(cell, 1e-6, 2000, False)
+    ref = µ.solvers.newton_cg(cell, [q["strain"]], solver, 1e-5, 1e-10, 0)[0]
+    nc = d * d
+    assert r["ok"] and r["converged"] == ref.success
+    assert r["newton_iterations"] == ref.nb_it and r["cg_iterations"] == ref.nb_fev
+    np.testing.assert_array_equal(r["strain"], np.asarray(ref.grad).reshape(-1, nc))
+    np.testing.assert_array_equal(r["stress"], np.asarray(ref.stress).reshape(-1, nc))
+    mean = np.asarray(ref.stress).reshape(-1, nc).mean(0)
+    assert np.abs(r["mean_stress"] - mean).max() <= 1e-13 * np.abs(mean).max()
+
+
+MIX = [[64, 64], [32, 32, 32], [128, 128], [48, 48, 48], [32, 32, 32], [64, 64],
+       [48, 48, 48], [32, 32, 32], [16, 16, 16], [32, 32, 32], [64, 64], [96, 32, 16]]
+
+
+def test_alone_equals_mix(serving):
+    reqs = [_request(r, i) for i, r in enumerate(MIX)]
+    alone = _alone(serving, reqs)
+    mixed, st = _serve(serving, reqs, max_batch=2, iters_per_launch=3)
+    for a, b in zip(alone, mixed):
+        assert a["ok"]
+        _same(a, b)
+    assert st["peak_running"] > 1 and st["batched_slots"] > st["batched_launches"]
+
+
+def test_failure_is_isolated(serving):
+    reqs = [_request(r, 40 + i) for i, r in enumerate(MIX[:6])]
+    reqs[2] = dict(reqs[2], maxiter=3)
+    alone = _alone(serving, reqs)
+    mixed, st = _serve(serving, reqs, max_batch=4, iters_per_launch=4)
+    assert not alone[2]["ok"] and alone[2]["status"] == 1  # ConvergenceError
+    for a, b in zip(alone, mixed):
+        _same(a, b)
+    assert st["failed"] == 1
+
+
+def test_graphs_and_direct_launches_agree(serving):
+    reqs = [_request(r, 70 + i) for i, r in enumerate(MIX[:5])]
+    g, _ = _serve(serving, reqs, max_batch=4, iters_per_launch=5, use_graphs=True)
+    d, _ = _serve(serving, reqs, max_batch=4, iters_per_launch=2, use_graphs=False)
+    for a, b in zip(g, d):
+        _same(a, b)
+
+
+def test_admission_keeps_budget(serving):
+    fp = serving.RVEServer.footprint([32, 32, 32])
+    budget = 2 * fp + fp // 2
+    reqs = [_request([32, 32, 32], 90 + i) for i in range(5)]
+    out, st = _serve(serving, reqs, hbm_budget=budget)
+    assert all(o["ok"] for o in out)
+    assert st["peak_running"] == 2 and st["peak_bytes"] <= budget
+    srv = serving.RVEServer(hbm_budget=budget)
+    big = _request([64, 64, 64], 1)
+    with pytest.raises(RuntimeError):
+        srv.submit(big["res"], big["E"], big["nu"], big["strain"])
+    with pytest.raises(RuntimeError):
+        q = _request([32, 32, 32], 2)
+        srv.submit(q["res"], q["E"], q["nu"], np.array([[0, 0.01, 0], [0, 0, 0], [0, 0, 0]]))
+    with pytest.raises(RuntimeError):
+        q = _request([16, 16, 16], 2)
+        srv.submit([15, 16, 16], np.ones(15 * 256), np.full(15 * 256, .3), q["strain"])
+    srv.close()
