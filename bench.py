@@ -19,6 +19,14 @@ Workloads (synthetic inputs generated from fixed seeds):
   c5  512^3 small strain LE4, E lognormal (median 50 GPa, sigma_log .8),
       nu ~ U[.15,.45], shear eps_xy = .01, CG tol 1e-8
   c1  51x51 small strain, disc of radius 10 (bin/small_case.py)
+  c4  serving stream: RVE requests, 3D with p = .7 (32^3, 48^3, 64^3, 96^3)
+      else 2D (64^2 .. 512^2), thresholded Gaussian-random-field phases
+      seeded per request, stiffness contrast log-uniform in [1, 100], nu in
+      [.2, .4], one symmetric strain with entries in +-.01, small strain,
+      FP32 + refined CG to 1e-6.  Phase 1 queues every request at t = 0
+      (saturated throughput, RVE solves/s); phase 2 replays a fresh stream
+      with Poisson arrivals at --load times that throughput (p50/p99
+      latency from each request's arrival time).
 """
 import argparse
 import ctypes
@@ -41,7 +49,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c5", "c1"])
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c5", "c1", "c4"])
+    ap.add_argument("--requests", type=int, default=1000,
+                    help="c4: requests per phase (the full stream is 10000)")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="c4: Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--size", type=int, default=0, help="override the grid size")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--kernel", default="auto",
@@ -199,7 +211,7 @@ class Workload:
 
 # -------------------------------------------------------- kernel accounting
 K_IDS = {"r2c": 0, "middle": 1, "axis0": 2, "c2r": 3, "material": 4, "update": 5}
-FAST = (16, 32, 64, 128, 256, 512)
+FAST = (16, 32, 48, 64, 96, 128, 256, 512)
 
 
 def kernel_name(wl, k):
@@ -363,12 +375,133 @@ def mixed_precision_error(mu, wl):
             "strain_measure": "F - I" if wl.form == "finite" else "eps"}
 
 
+# ------------------------------------------------------------ C4 serving
+C4_3D = [32, 48, 64, 96]
+C4_2D = [64, 128, 256, 512]
+
+
+def c4_request(seed):
+    """One request of the serving stream, from its own seed."""
+    rng = np.random.default_rng(seed)
+    if rng.random() < 0.7:
+        res = [int(rng.choice(C4_3D))] * 3
+    else:
+        res = [int(rng.choice(C4_2D))] * 2
+    d = len(res)
+    w = rng.standard_normal(res).astype(np.float32)
+    k2 = sum(np.meshgrid(*[np.fft.fftfreq(n).astype(np.float32) ** 2 for n in res],
+                         indexing="ij", sparse=True))
+    ell = 4.0 * res[0] / 32.0
+    ax = tuple(range(d))
+    f = np.fft.irfftn(np.fft.rfftn(w, axes=ax) *
+                      np.exp(-0.5 * (2 * np.pi) ** 2 * k2[..., :res[-1] // 2 + 1] * ell ** 2),
+                      s=res, axes=ax)
+    mask = (f > np.median(f)).ravel()
+    contrast = np.exp(rng.uniform(0.0, np.log(100.0)))
+    E = np.where(mask, 10e9 * contrast, 10e9)
+    nu = np.full(mask.size, rng.uniform(0.2, 0.4))
+    a = rng.uniform(-0.01, 0.01, (d, d))
+    return res, E, nu, 0.5 * (a + a.T)
+
+
+def c4_stream(n, seed0):
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=8) as ex:
+        return list(ex.map(c4_request, range(seed0, seed0 + n), chunksize=16))
+
+
+def c4_run(srv, reqs, arrivals):
+    """Submit each request at its arrival time (seconds after the start) and
+    turn the loop until all have finished; returns (wall seconds, latencies,
+    results)."""
+    n = len(reqs)
+    order = np.argsort(arrivals, kind="stable")
+    t_done = np.zeros(n)
+    idmap = {}
+    nxt = 0
+    finished = 0
+    t0 = time.perf_counter()
+    last_print = t0
+    while finished < n:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[order[nxt]] <= now:
+            i = order[nxt]
+            res, E, nu, eps = reqs[i]
+            idmap[srv.submit(res, E, nu, eps)] = i
+            nxt += 1
+        srv.progress(wait=False)
+        for rid in srv.poll():
+            t_done[idmap[rid]] = time.perf_counter() - t0
+            finished += 1
+        if time.perf_counter() - last_print > 20:
+            last_print = time.perf_counter()
+            print("bench: c4 {}/{} finished".format(finished, n), file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    lat = t_done - np.asarray(arrivals)
+    results = {i: srv.result(rid) for rid, i in idmap.items()}
+    for rid in idmap:
+        srv.release(rid)
+    return wall, lat, results
+
+
+def c4_main(args):
+    from muspectre_amd import serving
+    n = args.requests
+    print("bench: c4 generating {} + {} requests".format(n, n), file=sys.stderr, flush=True)
+    sat_reqs = c4_stream(n, 1)
+    poi_reqs = c4_stream(n, 100001)
+    srv = serving.RVEServer(max_batch=32, iters_per_launch=8, cg_tol=1e-6, cg_maxiter=4000)
+    # warm-up: one request of each resolution (plans, graphs)
+    warm = {}
+    for q in sat_reqs + poi_reqs:
+        warm.setdefault(tuple(q[0]), q)
+    c4_run(srv, list(warm.values()), np.zeros(len(warm)))
+    wall, lat, res = c4_run(srv, sat_reqs, np.zeros(n))
+    throughput = n / wall
+    voxel_iters = sum(int(np.prod(sat_reqs[i][0])) * r["cg_iterations"] for i, r in res.items())
+    failed = sum(1 for r in res.values() if not r["ok"])
+    print("bench: c4 saturated: {} requests in {:.2f} s, {:.1f} solves/s".format(n, wall, throughput),
+          file=sys.stderr, flush=True)
+    rate = args.load * throughput
+    arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, n))
+    wall2, lat2, res2 = c4_run(srv, poi_reqs, arr)
+    st = srv.stats()
+    srv.close()
+    p50, p99 = np.percentile(lat2, [50, 99])
+    out = {
+        "metric": "RVE solves/s",
+        "value": round(throughput, 2),
+        "unit": "RVE solves/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": round(1e3 * wall, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-operator/f64-refined",
+        "data": "synthetic",
+        "config": {"workload": "c4", "requests": n, "mix": "3D p=.7 {32,48,64,96}^3, 2D {64..512}^2",
+                   "cg_tol": 1e-6, "formulation": "small", "max_batch": 32, "iters_per_launch": 8},
+        "saturated": {"requests": n, "wall_s": round(wall, 3), "solves_per_s": round(throughput, 2),
+                      "voxel_cg_iters_per_s": round(voxel_iters / wall, 1), "failed": failed},
+        "poisson": {"rate_per_s": round(rate, 2), "load": args.load, "requests": n,
+                    "wall_s": round(wall2, 3), "solves_per_s": round(n / wall2, 2),
+                    "p50_ms": round(1e3 * p50, 2), "p99_ms": round(1e3 * p99, 2),
+                    "failed": sum(1 for r in res2.values() if not r["ok"])},
+        "server": st,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py runs on one GPU: every box here has one GPU and this build "
               "has no multi-GPU path (--gpus {} requested)".format(args.gpus), file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        return c4_main(args)
     import muspectre_amd as mu
     from muspectre_amd import _capi
     lib = _capi.load()
