@@ -922,13 +922,21 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
   int refs = 0;
   double rr = bnorm2;
   const double tol2 = tol * tol * bnorm2;
+  bool first = true;
   while (true) {
-    // true residual with the FP64 operator: r64 = b - A x
-    MUGPU_TRY(cell_operator64(c, d_x, c->r64));
-    MUGPU_TRY(axpby(MUGPU_FP64, 1.0, d_rhs, -1.0, c->r64, n));
-    MUGPU_TRY(dot_device(MUGPU_FP64, c->r64, c->r64, n, c->partials, c->scal, FIN_STORE_TMP1));
-    MUGPU_TRY(sync_scalars(c->scal));
-    rr = g_host_scal[SC_TMP1];
+    // true residual with the FP64 operator: r64 = b - A x; at the start
+    // x = 0, so A x = 0 exactly and r64 = b, |r64|^2 = |b|^2 bit for bit
+    if (first) {
+      MUGPU_TRY(convert(MUGPU_FP64, d_rhs, MUGPU_FP64, c->r64, n, 1.0));
+      rr = bnorm2;
+      first = false;
+    } else {
+      MUGPU_TRY(cell_operator64(c, d_x, c->r64));
+      MUGPU_TRY(axpby(MUGPU_FP64, 1.0, d_rhs, -1.0, c->r64, n));
+      MUGPU_TRY(dot_device(MUGPU_FP64, c->r64, c->r64, n, c->partials, c->scal, FIN_STORE_TMP1));
+      MUGPU_TRY(sync_scalars(c->scal));
+      rr = g_host_scal[SC_TMP1];
+    }
     ++refs;
     if (!(rr > tol2) || its >= maxiter || !(rr == rr)) break;
     // inner CG in FP32 on A d = r64 / |r64|

@@ -251,85 +251,6 @@ __device__ void jacobi3(const double* A, double* ev, double* V) {
   ev[2] = a[2][2];
 }
 
-// consistent tangent dP/dF of the HEP1 update (no reference counterpart; the
-// reference's coded tangent is not the derivative of P):
-//   K(i,m,j,n) = sum_k [sum_q (DL_ikjq + DL_ikqj) W_nq] Finv_mk - P_in Finv_mj
-// with W = F^-1 be*, DL = C : L, L the Daleckii-Krein derivative of ln at be*
-template <int D>
-__device__ void hep1_consistent_tangent(const double* Fi, const double* bes, const double* C,
-                                        const double* P, double* K) {
-  constexpr int NC = D * D;
-  double W[NC], ev[3], V[9];
-  matmul<D>(Fi, bes, W);
-  if (D == 3) {
-    jacobi3(bes, ev, V);
-  } else {
-    const double a = bes[0], b = 0.5 * (bes[2] + bes[1]), c = bes[3];
-    const double th = 0.5 * atan2(2 * b, a - c);
-    const double cs = cos(th), sn = sin(th);
-    V[0] = cs;
-    V[1] = sn;
-    V[2] = -sn;
-    V[3] = cs;
-    ev[0] = cs * cs * a + 2 * cs * sn * b + sn * sn * c;
-    ev[1] = sn * sn * a - 2 * cs * sn * b + cs * cs * c;
-  }
-  double g[NC];
-#pragma unroll
-  for (int m = 0; m < D; ++m)
-#pragma unroll
-    for (int n = 0; n < D; ++n) {
-      const double h = ev[n] - ev[m];
-      g[m + D * n] = h == 0.0 ? 1.0 / ev[m] : log1p(h / ev[m]) / h;
-    }
-  // L(q, c) with q = i + D j, c = k + D l; DL = C L computed column by column
-  for (int l = 0; l < D; ++l)
-    for (int k = 0; k < D; ++k) {
-      double Lc[NC];
-#pragma unroll
-      for (int j = 0; j < D; ++j)
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-          double s = 0;
-#pragma unroll
-          for (int m = 0; m < D; ++m)
-#pragma unroll
-            for (int n = 0; n < D; ++n)
-              s += g[m + D * n] * V[i + D * m] * V[j + D * n] * V[k + D * m] * V[l + D * n];
-          Lc[i + D * j] = s;
-        }
-      double* DLc = K + NC * (k + D * l);  // K holds DL until the last loop
-#pragma unroll
-      for (int r = 0; r < NC; ++r) {
-        double s = 0;
-#pragma unroll
-        for (int q = 0; q < NC; ++q) s += C[r + NC * q] * Lc[q];
-        DLc[r] = s;
-      }
-    }
-  // E(i,k,j,n) = sum_q (DL_ikjq + DL_ikqj) W_nq, then contract with Finv
-  double Ebuf[NC * NC];
-  for (int n = 0; n < D; ++n)
-    for (int j = 0; j < D; ++j)
-      for (int k = 0; k < D; ++k)
-        for (int i = 0; i < D; ++i) {
-          double e = 0;
-#pragma unroll
-          for (int q = 0; q < D; ++q)
-            e += (K[(i + D * k) + NC * (j + D * q)] + K[(i + D * k) + NC * (q + D * j)]) * W[n + D * q];
-          Ebuf[(i + D * k) + NC * (j + D * n)] = e;
-        }
-  for (int n = 0; n < D; ++n)
-    for (int j = 0; j < D; ++j)
-      for (int m = 0; m < D; ++m)
-        for (int i = 0; i < D; ++i) {
-          double s = 0;
-#pragma unroll
-          for (int k = 0; k < D; ++k) s += Ebuf[(i + D * k) + NC * (j + D * n)] * Fi[m + D * k];
-          K[(i + D * m) + NC * (j + D * n)] = s - P[i + D * n] * Fi[m + D * j];
-        }
-}
-
 struct MatArgs {
   const int* pix;
   int64_t n;
@@ -353,6 +274,100 @@ __device__ __forceinline__ void put_K(const MatArgs& a, int64_t v, int r, int c,
   } else {
     a.tan[tan_off(v, r + NC * c, NC * NC)] = val;
   }
+}
+
+// consistent tangent dP/dF of the HEP1 update (no reference counterpart; the
+// reference's coded tangent is not the derivative of P):
+//   K(i,m,j,o) = sum_k E(i,k,j,o) Finv_mk - P_io Finv_mj,
+//   E(.,.,j,o) = C : M_jo,  M_jo = sum_q W_oq [L(., (j,q)) + L(., (q,j))]
+// with W = F^-1 be*, L the Daleckii-Krein derivative of ln at be* and C the
+// algorithmic Kirchhoff modulus, which is c1 I(x)I + c2 Isymm + c3 N(x)N.
+// In the eigenbasis V of be* (g the divided differences of ln):
+//   M_jo = V [g o (V^T X V)] V^T,  X = e_j (x) W_o + W_o (x) e_j,
+// so each column (j, o) of K costs a few 3x3 products and no 81-entry array.
+template <int D>
+__device__ void hep1_consistent_tangent(const MatArgs& a, int64_t v, const double* Fi,
+                                        const double* bes, double c1, double c2, double c3,
+                                        const double* Ns, const double* P) {
+  constexpr int NC = D * D;
+  double W[NC], ev[3], V[9];
+  matmul<D>(Fi, bes, W);
+  if (D == 3) {
+    jacobi3(bes, ev, V);
+  } else {
+    const double a0 = bes[0], b0 = 0.5 * (bes[2] + bes[1]), c0 = bes[3];
+    const double th = 0.5 * atan2(2 * b0, a0 - c0);
+    const double cs = cos(th), sn = sin(th);
+    V[0] = cs;
+    V[1] = sn;
+    V[2] = -sn;
+    V[3] = cs;
+    ev[0] = cs * cs * a0 + 2 * cs * sn * b0 + sn * sn * c0;
+    ev[1] = sn * sn * a0 - 2 * cs * sn * b0 + cs * cs * c0;
+  }
+  double g[NC];
+#pragma unroll
+  for (int m = 0; m < D; ++m)
+#pragma unroll
+    for (int n = 0; n < D; ++n) {
+      const double h = ev[n] - ev[m];
+      g[m + D * n] = h == 0.0 ? 1.0 / ev[m] : log1p(h / ev[m]) / h;
+    }
+  // w(o, m) = sum_q W(o, q) V(q, m)
+  double w[NC];
+#pragma unroll
+  for (int m = 0; m < D; ++m)
+#pragma unroll
+    for (int o = 0; o < D; ++o) {
+      double s = 0;
+#pragma unroll
+      for (int q = 0; q < D; ++q) s += W[o + D * q] * V[q + D * m];
+      w[o + D * m] = s;
+    }
+  for (int o = 0; o < D; ++o)
+    for (int j = 0; j < D; ++j) {
+      double Z[NC], M[NC], E[NC];
+#pragma unroll
+      for (int n = 0; n < D; ++n)
+#pragma unroll
+        for (int m = 0; m < D; ++m)
+          Z[m + D * n] =
+              g[m + D * n] * (V[j + D * m] * w[o + D * n] + w[o + D * m] * V[j + D * n]);
+#pragma unroll
+      for (int bb = 0; bb < D; ++bb)
+#pragma unroll
+        for (int aa = 0; aa < D; ++aa) {
+          double s = 0;
+#pragma unroll
+          for (int n = 0; n < D; ++n) {
+            double t = 0;
+#pragma unroll
+            for (int m = 0; m < D; ++m) t += V[aa + D * m] * Z[m + D * n];
+            s += t * V[bb + D * n];
+          }
+          M[aa + D * bb] = s;
+        }
+      double tr = 0, nm = 0;
+#pragma unroll
+      for (int aa = 0; aa < D; ++aa) tr += M[aa * (D + 1)];
+#pragma unroll
+      for (int q = 0; q < NC; ++q) nm += Ns[q] * M[q];
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+          E[r + D * k] = (r == k ? c1 * tr : 0.0) + c2 * 0.5 * (M[r + D * k] + M[k + D * r]) +
+                         c3 * Ns[r + D * k] * nm;
+#pragma unroll
+      for (int m = 0; m < D; ++m)
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+          double s = 0;
+#pragma unroll
+          for (int k = 0; k < D; ++k) s += E[i + D * k] * Fi[m + D * k];
+          put_K<D>(a, v, i + D * m, j + D * o, s - P[i + D * o] * Fi[m + D * j]);
+        }
+    }
 }
 
 // write an isotropic tangent (lambda, mu) in the cell's representation
@@ -511,30 +526,16 @@ __global__ __launch_bounds__(256) void k_hep1(MatArgs a) {
   const double a0 = plastic ? dg * mu / taueq : 0.0;
   const double a1 = mu / (a.H + 3 * mu);
   if (CT) {
-    // algorithmic modulus C(r,m,n,s) as the nc x nc T4 matrix
-    double C[NC * NC];
-    for (int s = 0; s < D; ++s)
-      for (int n = 0; n < D; ++n)
-        for (int m = 0; m < D; ++m)
-          for (int r = 0; r < D; ++r) {
-            double c;
-            if (plastic) {
-              const double itrac = (r == m && n == s) ? 1.0 : 0.0;
-              const double isym = 0.5 * (((r == n && m == s) ? 1.0 : 0.0) + ((r == s && m == n) ? 1.0 : 0.0));
-              c = (a.bulk / 2. - mu / 3 + a0 * mu) * itrac + (1 - 3 * a0) * mu * isym +
-                  2 * mu * (a0 - a1) * Ns[r + D * m] * Ns[n + D * s];
-            } else {
-              c = 0.5 * iso_C<D>(lam, mu, r, m, n, s);
-            }
-            C[(r + D * m) + NC * (n + D * s)] = c;
-          }
+    // algorithmic modulus C = c1 Itrac + c2 Isymm + c3 N (x) N: plastic
+    // (K/2 - mu/3 + a0 mu, (1 - 3 a0) mu, 2 mu (a0 - a1)), elastic half the
+    // Hooke tensor (lambda/2, mu, 0) (material_hyper_elasto_plastic1.hh:299-308)
+    const double c1 = plastic ? a.bulk / 2. - mu / 3 + a0 * mu : 0.5 * lam;
+    const double c2 = plastic ? (1 - 3 * a0) * mu : mu;
+    const double c3 = plastic ? 2 * mu * (a0 - a1) : 0.0;
     double Pl[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) Pl[c] = a.P[v * NC + c];
-    double K[NC * NC];
-    hep1_consistent_tangent<D>(Fi, bes, C, Pl, K);
-    for (int c = 0; c < NC; ++c)
-      for (int r = 0; r < NC; ++r) put_K<D>(a, v, r, c, K[r + NC * c]);
+    hep1_consistent_tangent<D>(a, v, Fi, bes, c1, c2, c3, Ns, Pl);
     return;
   }
   // Kirchhoff tangent C(r,m,n,s), then M(i,m,n) = sum_r Finv(i,r) sum_s C(r,m,n,s)

@@ -329,6 +329,7 @@ struct mugpu_server_s {
   int start_resid(Request* r);
   int finish(Request* r);
   int after_inner(Request* r);
+  int residual_known(Request* r);
   int launch_chunk(ResClass* c);
   int process_chunk(ResClass* c);
   int graph_for(ResClass* c, int cap, hipGraphExec_t* out);
@@ -528,6 +529,36 @@ int mugpu_server_s::finish(Request* r) {
   return MUGPU_OK;
 }
 
+// the refinement step with the residual |r64|^2 = r->rr known (the loop of
+// the mixed-precision solve: stop, or start the next inner CG)
+int mugpu_server_s::residual_known(Request* r) {
+  ++r->refs;
+  if (!(r->rr > r->tol2) || r->its >= r->maxiter || !(r->rr == r->rr)) {
+    r->counter += r->its;
+    if (r->rr > r->tol2 || !(r->rr == r->rr)) {
+      // ConvergenceError: the request fails alone
+      r->status = MUGPU_ECONVERGENCE;
+      ++stats[3];
+      return finish(r);
+    }
+    // F += dF, |dF|, |F| (solvers.cc:185-200)
+    set_current_stream(sr);
+    MUGPU_TRY(axpby(MUGPU_FP64, 1.0, r->x, 1.0, r->F, r->nd));
+    MUGPU_TRY(dot_device(MUGPU_FP64, r->x, r->x, r->nd, r->part, r->scal, FIN_STORE_TMP0));
+    MUGPU_TRY(dot_device(MUGPU_FP64, r->F, r->F, r->nd, r->part, r->scal, FIN_STORE_TMP1));
+    r->st = ST_WAIT_NORMS;
+    return record(r);
+  }
+  r->s = 1.0 / std::sqrt(r->rr);
+  set_current_stream(sr);
+  MUGPU_HIP(hipMemsetAsync(r->x32, 0, sizeof(float) * r->nd, sr));
+  MUGPU_TRY(convert(MUGPU_FP64, r->r64, MUGPU_FP32, r->r32, r->nd, -r->s));
+  MUGPU_TRY(convert(MUGPU_FP64, r->r64, MUGPU_FP32, r->p32, r->nd, r->s));
+  MUGPU_TRY(dot_device(MUGPU_FP32, r->r32, r->r32, r->nd, r->part, r->scal, FIN_STORE_RDR));
+  r->st = ST_WAIT_RDR;
+  return record(r);
+}
+
 // the device work of r's current stage has completed: take the next step
 int mugpu_server_s::advance(Request* r) {
   const double* h = r->h;
@@ -549,35 +580,15 @@ int mugpu_server_s::advance(Request* r) {
       r->its = 0;
       r->refs = 0;
       r->tol2 = tol * tol * r->bnorm2;
-      return start_resid(r);
-    case ST_WAIT_RESID: {
-      r->rr = h[SC_TMP1];
-      ++r->refs;
-      if (!(r->rr > r->tol2) || r->its >= r->maxiter || !(r->rr == r->rr)) {
-        r->counter += r->its;
-        if (r->rr > r->tol2 || !(r->rr == r->rr)) {
-          // ConvergenceError: the request fails alone
-          r->status = MUGPU_ECONVERGENCE;
-          ++stats[3];
-          return finish(r);
-        }
-        // F += dF, |dF|, |F| (solvers.cc:185-200)
-        set_current_stream(sr);
-        MUGPU_TRY(axpby(MUGPU_FP64, 1.0, r->x, 1.0, r->F, r->nd));
-        MUGPU_TRY(dot_device(MUGPU_FP64, r->x, r->x, r->nd, r->part, r->scal, FIN_STORE_TMP0));
-        MUGPU_TRY(dot_device(MUGPU_FP64, r->F, r->F, r->nd, r->part, r->scal, FIN_STORE_TMP1));
-        r->st = ST_WAIT_NORMS;
-        return record(r);
-      }
-      r->s = 1.0 / std::sqrt(r->rr);
+      // first residual at x = 0: A x = 0 exactly, so r64 = b and
+      // |r64|^2 = |b|^2 bit for bit (no operator application)
       set_current_stream(sr);
-      MUGPU_HIP(hipMemsetAsync(r->x32, 0, sizeof(float) * r->nd, sr));
-      MUGPU_TRY(convert(MUGPU_FP64, r->r64, MUGPU_FP32, r->r32, r->nd, -r->s));
-      MUGPU_TRY(convert(MUGPU_FP64, r->r64, MUGPU_FP32, r->p32, r->nd, r->s));
-      MUGPU_TRY(dot_device(MUGPU_FP32, r->r32, r->r32, r->nd, r->part, r->scal, FIN_STORE_RDR));
-      r->st = ST_WAIT_RDR;
-      return record(r);
-    }
+      MUGPU_TRY(convert(MUGPU_FP64, r->rhs, MUGPU_FP64, r->r64, r->nd, 1.0));
+      r->rr = r->bnorm2;
+      return residual_known(r);
+    case ST_WAIT_RESID:
+      r->rr = h[SC_TMP1];
+      return residual_known(r);
     case ST_WAIT_RDR: {
       const double rdr = h[SC_RDR];
       const double tol2i = cfg.inner_tol * cfg.inner_tol * rdr;
