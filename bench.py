@@ -398,10 +398,17 @@ def c4_request(seed):
                       s=res, axes=ax)
     mask = (f > np.median(f)).ravel()
     contrast = np.exp(rng.uniform(0.0, np.log(100.0)))
-    E = np.where(mask, 10e9 * contrast, 10e9)
-    nu = np.full(mask.size, rng.uniform(0.2, 0.4))
+    nu = rng.uniform(0.2, 0.4)
     a = rng.uniform(-0.01, 0.01, (d, d))
-    return res, E, nu, 0.5 * (a + a.T)
+    # phases kept as a mask; the per-voxel moduli are formed at submission
+    return res, np.packbits(mask), 10e9 * contrast, 10e9, nu, 0.5 * (a + a.T)
+
+
+def c4_arrays(q):
+    res, bits, e_in, e_mat, nu, eps = q
+    N = int(np.prod(res))
+    mask = np.unpackbits(bits, count=N).astype(bool)
+    return res, np.where(mask, e_in, e_mat), np.full(N, nu), eps
 
 
 def c4_stream(n, seed0):
@@ -426,7 +433,7 @@ def c4_run(srv, reqs, arrivals):
         now = time.perf_counter() - t0
         while nxt < n and arrivals[order[nxt]] <= now:
             i = order[nxt]
-            res, E, nu, eps = reqs[i]
+            res, E, nu, eps = c4_arrays(reqs[i])
             idmap[srv.submit(res, E, nu, eps)] = i
             nxt += 1
         srv.progress(wait=False)
