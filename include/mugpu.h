@@ -312,6 +312,19 @@ int mugpu_server_stats(mugpu_server server, int64_t *out, int n);
 int mugpu_timing_start(int kernel_class);
 int mugpu_timing_stop(int64_t *launches, double *total_ms);
 int mugpu_timing_class(int kernel_class, int64_t *launches, double *total_ms);
+/* autotuner measurement: average duration in ms of one compile-time-length
+ * FFT pass (MUGPU_PASS_*) in one variant (0: 16 or 12 values per thread,
+ * 1: 8 values per thread) over `reps` launches on zeroed scratch buffers of
+ * the grid `resolutions`; iso selects the isotropic tangent for the r2c
+ * pass.  The chosen variants are compiled into the library
+ * (muspectre_amd/csrc/fft_tuned.inc, written by muspectre_amd/autotune.py). */
+#define MUGPU_PASS_R2C 0          /* last-axis r2c with p update and K:p      */
+#define MUGPU_PASS_MIDDLE 1       /* middle-axis c2c (3D)                     */
+#define MUGPU_PASS_AXIS0 2        /* first axis with the projection           */
+#define MUGPU_PASS_C2R 3          /* last-axis c2r with the p.Ap partials     */
+#define MUGPU_PASS_AXIS0_PLAIN 4  /* first-axis c2c of the engine transforms  */
+int mugpu_fft_tune_pass(int dim, const int *resolutions, int precision, int pass,
+                        int variant, int iso, int reps, double *avg_ms);
 /* streaming device copy of `bytes` (multiple of 16), the bandwidth
  * reference for the roofline and for the HBM counter calibration */
 int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);

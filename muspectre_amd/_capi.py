@@ -85,6 +85,7 @@ SIGNATURES = [
     ("mugpu_timing_start", _int, [_int]),
     ("mugpu_timing_stop", _int, [_i64p, _dp]),
     ("mugpu_timing_class", _int, [_int, _i64p, _dp]),
+    ("mugpu_fft_tune_pass", _int, [_int, _ip, _int, _int, _int, _int, _int, _dp]),
     ("mugpu_server_create", _int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(_vp)]),
     ("mugpu_server_destroy", _int, [_vp]),
     ("mugpu_server_footprint", _int, [_int, _ip, ctypes.POINTER(ctypes.c_size_t)]),

@@ -338,6 +338,101 @@ extern "C" int mugpu_timing_class(int kernel_class, int64_t* launches, double* t
   return MUGPU_OK;
 }
 
+// time one compile-time-length pass in one variant on scratch buffers of
+// the grid res (the autotuner's measurement)
+extern "C" int mugpu_fft_tune_pass(int dim, const int* res, int precision, int pass, int variant,
+                                   int iso, int reps, double* avg_ms) {
+  MUGPU_TRY(ensure_init());
+  if (!res || !avg_ms || reps < 1 || pass < 0 || pass >= FV_COUNT || variant < 0 || variant > 1)
+    return MUGPU_EARG;
+  const int axis = (pass == FV_R2C || pass == FV_C2R) ? dim - 1 : pass == FV_STRIDED1 ? 1 : 0;
+  if ((pass == FV_STRIDED1 && dim != 3) || !fast_size(res[axis]) || (variant && res[axis] % 3 == 0)) {
+    set_error("no variant %d of pass %d for an axis of %d points", variant, pass, res[axis]);
+    return MUGPU_EARG;
+  }
+  FftPlan plan;
+  const int nc = dim * dim;
+  int rc = plan_init(plan, dim, res, nc, precision);
+  if (rc) {
+    plan_free(plan);
+    return rc;
+  }
+  const int pr = precision == MUGPU_FP64 ? 0 : 1;
+  for (int i = 0; i < 2; ++i) plan.fvar[pr][pass][i] = (unsigned char)variant;
+  const size_t s = precision == MUGPU_FP64 ? 8 : 4;
+  const int tk = iso ? TAN_ISO : (dim == 3 ? TAN_SYM : TAN_FULL);
+  const size_t nd = (size_t)plan.N * nc;
+  void *spec = nullptr, *a = nullptr, *b = nullptr, *c = nullptr, *tan = nullptr;
+  double* scal = nullptr;
+  bool ok = hipMalloc(&spec, 16 * (size_t)plan.kpts * nc) == hipSuccess &&
+            hipMalloc(&a, s * nd) == hipSuccess && hipMalloc(&b, s * nd) == hipSuccess &&
+            hipMalloc(&c, s * nd) == hipSuccess &&
+            hipMalloc(&tan, s * (size_t)tangent_values(tk, dim, plan.N)) == hipSuccess &&
+            hipMalloc((void**)&scal, sizeof(double) * SC_COUNT) == hipSuccess;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ok) {
+    ok = hipMemsetAsync(spec, 0, 16 * (size_t)plan.kpts * nc, g_stream) == hipSuccess &&
+         hipMemsetAsync(a, 0, s * nd, g_stream) == hipSuccess &&
+         hipMemsetAsync(b, 0, s * nd, g_stream) == hipSuccess &&
+         hipMemsetAsync(c, 0, s * nd, g_stream) == hipSuccess &&
+         hipMemsetAsync(tan, 0, s * (size_t)tangent_values(tk, dim, plan.N), g_stream) == hipSuccess &&
+         hipMemsetAsync(scal, 0, sizeof(double) * SC_COUNT, g_stream) == hipSuccess &&
+         hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  }
+  if (!ok) {
+    set_error("scratch allocation for the pass timing failed");
+    rc = MUGPU_EMEMORY;
+  }
+  auto run = [&]() -> int {
+    switch (pass) {
+      case FV_R2C: {
+        OpSpec op;
+        op.in_mode = IN_TAN_PUPDATE;
+        op.in = a;
+        op.rvec = b;
+        op.pout = a;
+        op.scal = scal;
+        op.tan_kind = tk;
+        op.tangent = tan;
+        return fast_r2c(plan, precision, op, spec);
+      }
+      case FV_C2R: {
+        OpSpec op;
+        op.out_mode = OUT_DOT;
+        op.out = c;
+        op.dotv = a;
+        return fast_c2r(plan, precision, op, spec);
+      }
+      case FV_AXIS0: {
+        ProjParams pj;
+        pj.kind = MUGPU_PROJ_SMALL_STRAIN;
+        pj.allow_even = 1;
+        pj.norm = 1.0 / (double)plan.N;
+        return fast_axis0_project(plan, precision, pj, spec);
+      }
+      case FV_STRIDED1: return fast_strided(plan, precision, 1, spec, -1);
+      default: return fast_strided(plan, precision, 0, spec, -1);
+    }
+  };
+  if (rc == MUGPU_OK) rc = run();
+  if (rc == MUGPU_OK && hipEventRecord(e0, g_stream) == hipSuccess) {
+    for (int i = 0; i < reps && rc == MUGPU_OK; ++i) rc = run();
+    if (rc == MUGPU_OK && hipEventRecord(e1, g_stream) == hipSuccess &&
+        hipEventSynchronize(e1) == hipSuccess) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      *avg_ms = ms / reps;
+    }
+  }
+  (void)hipStreamSynchronize(g_stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (void* q : {spec, a, b, c, tan, (void*)scal})
+    if (q) (void)hipFree(q);
+  plan_free(plan);
+  return rc;
+}
+
 extern "C" int mugpu_copy(void* d_dst, const void* d_src, size_t bytes) {
   MUGPU_TRY(ensure_init());
   if (bytes % 16) {

@@ -6,7 +6,8 @@
 //    P = n/E threads per line; thread (l, p) holds E complex values of line
 //    l in registers (E = 16, or 12 for the lengths with a factor 3).  Consecutive lanes work on consecutive lines, so
 //    every LDS access of a pass is unit stride across lanes (element m of
-//    line l at s[m * LS + l], LS = L + 1), and on the strided axes every HBM
+//    line l at s[m * LS + l], LS = L or L + 1, whichever is odd), and on
+//    the strided axes every HBM
 //    access is a contiguous run of L values;
 //  * register-resident Stockham passes of radix 16/8/4/3/2 with every index
 //    a compile-time shift or mask (the span before the radix-3 pass is a
@@ -28,46 +29,55 @@ bool fast_size(int n) {
 
 namespace {
 
-// complex values per thread and threads per line
-__host__ __device__ constexpr int fe(int n) { return n % 3 == 0 ? 12 : 16; }
-__host__ __device__ constexpr int fp(int n) { return n / fe(n); }
+// complex values per thread: E = 16 (12 for the lengths with a factor 3), or
+// E = 8 with twice the threads per line (n / E) on the same tile; ft(n) is
+// the threads per line of the default plan, which fixes the tile shapes
+__host__ __device__ constexpr int fe_default(int n) { return n % 3 == 0 ? 12 : 16; }
+__host__ __device__ constexpr int ft(int n) { return n / fe_default(n); }
 
-// radices of the passes: each divides fe(n), and the span before every pass
-// is a power of two
-__host__ __device__ constexpr int fast_npass(int n) {
-  return n == 16 ? 1 : n == 512 || n == 48 ? 3 : n == 96 ? 4 : 2;
+// radices of the passes of length n with E values per thread: each divides
+// E, and the span before every pass is a power of two
+__host__ __device__ constexpr int fast_npass(int n, int e) {
+  return e == 8 ? (n == 16 || n == 32 || n == 64 ? 2 : 3)
+                : (n == 16 ? 1 : n == 512 || n == 48 ? 3 : n == 96 ? 4 : 2);
 }
-__host__ __device__ constexpr int fast_radix(int n, int i) {
-  return n == 16    ? 16
-         : n == 32  ? (i == 0 ? 8 : 4)
-         : n == 48  ? (i < 2 ? 4 : 3)
-         : n == 64  ? 8
-         : n == 96  ? (i < 2 ? 4 : i == 2 ? 2 : 3)
-         : n == 128 ? (i == 0 ? 16 : 8)
-         : n == 256 ? 16
-                    : 8;
+__host__ __device__ constexpr int fast_radix(int n, int e, int i) {
+  return e == 8 ? (n == 16    ? (i == 0 ? 8 : 2)
+                   : n == 32  ? (i == 0 ? 8 : 4)
+                   : n == 64  ? 8
+                   : n == 128 ? (i < 2 ? 8 : 2)
+                   : n == 256 ? (i < 2 ? 8 : 4)
+                              : 8)
+                : (n == 16    ? 16
+                   : n == 32  ? (i == 0 ? 8 : 4)
+                   : n == 48  ? (i < 2 ? 4 : 3)
+                   : n == 64  ? 8
+                   : n == 96  ? (i < 2 ? 4 : i == 2 ? 2 : 3)
+                   : n == 128 ? (i == 0 ? 16 : 8)
+                   : n == 256 ? 16
+                              : 8);
 }
 
-template <int N, int R>
+template <int N, int E, int R>
 __device__ __forceinline__ int in_idx(int p, int b, int t) {
-  return p + fp(N) * b + t * (N / R);
+  return p + (N / E) * b + t * (N / R);
 }
-template <int N, int R, int NS>
+template <int N, int E, int R, int NS>
 __device__ __forceinline__ int out_idx(int p, int b, int t) {
-  const int j = p + fp(N) * b;
+  const int j = p + (N / E) * b;
   return (j / NS) * NS * R + (j & (NS - 1)) + t * NS;
 }
 
-// twiddles and butterflies of pass (R, NS) on this thread's fe(N) values
-template <typename T, int N, int R, int NS, int SG>
+// twiddles and butterflies of pass (R, NS) on this thread's E values
+template <typename T, int N, int E, int R, int NS, int SG>
 __device__ __forceinline__ void bfly_pass(c_t<T>* v, int p, const c_t<T>* __restrict__ tw) {
-  constexpr int BPT = fe(N) / R;
-  static_assert(BPT * R == fe(N) && (NS & (NS - 1)) == 0, "radix plan");
+  constexpr int BPT = E / R;
+  static_assert(BPT * R == E && (NS & (NS - 1)) == 0, "radix plan");
   constexpr int TWS = N / (NS * R);
 #pragma unroll
   for (int b = 0; b < BPT; ++b) {
     if (NS > 1) {
-      const int k = (p + fp(N) * b) & (NS - 1);
+      const int k = (p + (N / E) * b) & (NS - 1);
 #pragma unroll
       for (int t = 1; t < R; ++t) {
         c_t<T> w = tw[t * k * TWS];
@@ -83,52 +93,52 @@ __device__ __forceinline__ void bfly_pass(c_t<T>* v, int p, const c_t<T>* __rest
 // values go through the workgroup's LDS lines (at(m) is element m of this
 // thread's line).  On return v holds the outputs of the last pass, to be
 // stored at out_idx<N, R_last, N / R_last>.
-template <typename T, int N, int SG, int I, int NS>
+template <typename T, int N, int E, int SG, int I, int NS>
 struct Passes {
-  static constexpr int R = fast_radix(N, I);
+  static constexpr int R = fast_radix(N, E, I);
   template <class At>
   __device__ static void run(c_t<T>* v, int p, const c_t<T>* __restrict__ tw, At at) {
-    bfly_pass<T, N, R, NS, SG>(v, p, tw);
-    if constexpr (I + 1 < fast_npass(N)) {
-      constexpr int R2 = fast_radix(N, I + 1);
+    bfly_pass<T, N, E, R, NS, SG>(v, p, tw);
+    if constexpr (I + 1 < fast_npass(N, E)) {
+      constexpr int R2 = fast_radix(N, E, I + 1);
 #pragma unroll
-      for (int b = 0; b < fe(N) / R; ++b)
+      for (int b = 0; b < E / R; ++b)
 #pragma unroll
-        for (int t = 0; t < R; ++t) at(out_idx<N, R, NS>(p, b, t)) = v[b * R + t];
+        for (int t = 0; t < R; ++t) at(out_idx<N, E, R, NS>(p, b, t)) = v[b * R + t];
       __syncthreads();
 #pragma unroll
-      for (int b = 0; b < fe(N) / R2; ++b)
+      for (int b = 0; b < E / R2; ++b)
 #pragma unroll
-        for (int t = 0; t < R2; ++t) v[b * R2 + t] = at(in_idx<N, R2>(p, b, t));
+        for (int t = 0; t < R2; ++t) v[b * R2 + t] = at(in_idx<N, E, R2>(p, b, t));
       __syncthreads();
-      Passes<T, N, SG, I + 1, NS * R>::run(v, p, tw, at);
+      Passes<T, N, E, SG, I + 1, NS * R>::run(v, p, tw, at);
     }
   }
 };
 
-template <int N>
+template <int N, int E>
 struct Last {
-  static constexpr int R = fast_radix(N, fast_npass(N) - 1);
+  static constexpr int R = fast_radix(N, E, fast_npass(N, E) - 1);
   static constexpr int NS = N / R;
 };
 
 // load the inputs of the first pass: v[b R0 + t] = ld(in index)
-template <typename T, int N, class Ld>
+template <typename T, int N, int E, class Ld>
 __device__ __forceinline__ void load_first(c_t<T>* v, int p, Ld ld) {
-  constexpr int R0 = fast_radix(N, 0);
+  constexpr int R0 = fast_radix(N, E, 0);
 #pragma unroll
-  for (int b = 0; b < fe(N) / R0; ++b)
+  for (int b = 0; b < E / R0; ++b)
 #pragma unroll
-    for (int t = 0; t < R0; ++t) v[b * R0 + t] = ld(in_idx<N, R0>(p, b, t));
+    for (int t = 0; t < R0; ++t) v[b * R0 + t] = ld(in_idx<N, E, R0>(p, b, t));
 }
 // store the outputs of the last pass: st(out index, value)
-template <typename T, int N, class St>
+template <typename T, int N, int E, class St>
 __device__ __forceinline__ void store_last(const c_t<T>* v, int p, St st) {
-  constexpr int R = Last<N>::R, NS = Last<N>::NS;
+  constexpr int R = Last<N, E>::R, NS = Last<N, E>::NS;
 #pragma unroll
-  for (int b = 0; b < fe(N) / R; ++b)
+  for (int b = 0; b < E / R; ++b)
 #pragma unroll
-    for (int t = 0; t < R; ++t) st(out_idx<N, R, NS>(p, b, t), v[b * R + t]);
+    for (int t = 0; t < R; ++t) st(out_idx<N, E, R, NS>(p, b, t), v[b * R + t]);
 }
 
 template <typename T>
@@ -155,26 +165,34 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
 // last-axis passes: rows per workgroup (even: two rows share a complex
 // line); 288 threads in 3D, 256 in 2D
 __host__ __device__ constexpr int z_rows(int n, int dim) {
-  return dim == 3 ? 64 / fp(n) : 128 / fp(n);
+  return dim == 3 ? 64 / ft(n) : 128 / ft(n);
 }
-// first-axis projection pass: k-points per workgroup (same thread counts)
-__host__ __device__ constexpr int x_kpts(int n, int dim) {
-  return dim == 3 ? 32 / fp(n) : 64 / fp(n);
+// first-axis projection pass: k-points per workgroup (same thread counts in
+// FP64; FP32 takes twice as many for the same LDS and longer HBM runs)
+__host__ __device__ constexpr int x_kpts(int n, int dim, bool f32, int e) {
+  // at most 1024 threads per workgroup
+  return (dim == 3 ? 32 / ft(n) : 64 / ft(n)) * (f32 ? 2 : 1) * dim * dim * (n / e) <= 1024
+             ? (dim == 3 ? 32 / ft(n) : 64 / ft(n)) * (f32 ? 2 : 1)
+             : (dim == 3 ? 32 / ft(n) : 64 / ft(n));
 }
+// LDS line stride: the line count rounded up to an odd number (lanes on
+// consecutive lines then spread over the banks, and no padding is spent when
+// the count is already odd)
+__host__ __device__ constexpr int lstride(int L) { return L | 1; }
 // strided passes: lines per workgroup (a run of 256 bytes)
-template <typename T>
+template <typename T, int N, int E>
 constexpr int s_lines() {
-  return 256 / (int)sizeof(c_t<T>);
+  return 256 / (int)sizeof(c_t<T>) * (N / E) <= 1024 ? 256 / (int)sizeof(c_t<T>) : 1024 / (N / E);
 }
 
 // ---- strided c2c (middle axis, or first axis without projection) ----------
 // data[(o * N + m) * inner + w]: workgroup (w-tile, o) transforms lines
 // w0 .. w0 + L - 1 of plane o
-template <typename T, int N, int SG>
+template <typename T, int N, int E, int SG>
 __device__ __forceinline__ void strided_body(c_t<T>* __restrict__ data, int64_t inner,
                                              const c_t<T>* __restrict__ tw, unsigned char* smem,
                                              int64_t bx, int64_t by, int64_t gx, int64_t gy) {
-  constexpr int L = s_lines<T>(), LS = L + 1;
+  constexpr int L = s_lines<T, N, E>(), LS = lstride(L);
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   const int64_t t = xcd_tile(by * gx + bx, gx * gy);
@@ -182,27 +200,27 @@ __device__ __forceinline__ void strided_body(c_t<T>* __restrict__ data, int64_t 
   const int64_t w = (t - o * gx) * L + l;
   const bool valid = w < inner;
   c_t<T>* base = data + o * N * inner + w;
-  c_t<T> v[fe(N)];
-  load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
-  Passes<T, N, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
-  if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
+  c_t<T> v[E];
+  load_first<T, N, E>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
+  Passes<T, N, E, SG, 0, 1>::run(v, p, tw, [&](int m) -> c_t<T>& { return s[m * LS + l]; });
+  if (valid) store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
 }
 
-template <typename T, int N, int SG>
-__global__ __launch_bounds__(s_lines<T>() * fp(N)) void k_strided_fast(
+template <typename T, int N, int E, int SG>
+__global__ __launch_bounds__((s_lines<T, N, E>() * (N / E))) void k_strided_fast(
     c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem[];
-  strided_body<T, N, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+  strided_body<T, N, E, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 // batched (FP32, one RVE per blockIdx.z)
-template <int N, int SG>
-__global__ __launch_bounds__(s_lines<float>() * fp(N)) void k_strided_batch(
+template <int N, int E, int SG>
+__global__ __launch_bounds__((s_lines<float, N, E>() * (N / E))) void k_strided_batch(
     const BatchSlot* __restrict__ slots, int64_t inner, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  strided_body<float, N, SG>(sl.spec, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x,
+  strided_body<float, N, E, SG>(sl.spec, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x,
                              gridDim.y);
 }
 
@@ -220,10 +238,11 @@ struct XArgs {
   T norm;
 };
 
-template <typename T, int N, int DIM>
+template <typename T, int N, int E, int DIM>
 __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* smem, int64_t bx,
                                            int64_t gx) {
-  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM), L = KP * NC, LS = L + 1, NT = L * fp(N);
+  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM, sizeof(T) == 4, E), L = KP * NC, LS = lstride(L),
+                NT = L * (N / E);
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   const int64_t q0 = xcd_tile(bx, gx) * KP;
@@ -232,10 +251,10 @@ __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* sme
   const bool valid = w < inner;
   c_t<T>* base = a.data + w;
   auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
-  c_t<T> v[fe(N)];
-  load_first<T, N>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
-  Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
-  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  c_t<T> v[E];
+  load_first<T, N, E>(v, p, [&](int m) { return valid ? base[m * inner] : czero<T>(); });
+  Passes<T, N, E, -1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
   __syncthreads();
   // projection per wave vector (k0, q0 + kp)
   for (int e = threadIdx.x; e < N * KP; e += NT) {
@@ -260,20 +279,20 @@ __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* sme
     project_point<T, DIM>(f, 1, xi, a.kind, a.norm);
   }
   __syncthreads();
-  load_first<T, N>(v, p, [&](int m) { return at(m); });
+  load_first<T, N, E>(v, p, [&](int m) { return at(m); });
   __syncthreads();
-  Passes<T, N, +1, 0, 1>::run(v, p, a.tw, at);
-  if (valid) store_last<T, N>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
+  Passes<T, N, E, +1, 0, 1>::run(v, p, a.tw, at);
+  if (valid) store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
 }
 
-template <typename T, int N, int DIM>
-__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_fast(XArgs<T> a) {
+template <typename T, int N, int E, int DIM>
+__global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  axis0_body<T, N, DIM>(a, smem, blockIdx.x, gridDim.x);
+  axis0_body<T, N, E, DIM>(a, smem, blockIdx.x, gridDim.x);
 }
 
-template <int N, int DIM>
-__global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_batch(
+template <int N, int E, int DIM>
+__global__ __launch_bounds__(x_kpts(N, DIM, true, E) * DIM * DIM * (N / E)) void k_axis0_batch(
     const BatchSlot* __restrict__ slots, XArgs<float> shape) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -284,7 +303,7 @@ __global__ __launch_bounds__(x_kpts(N, DIM) * DIM * DIM * fp(N)) void k_axis0_ba
   for (int d = 0; d < 3; ++d) a.L[d] = sl.L[d];
   a.norm = (float)sl.norm;
   extern __shared__ __align__(16) unsigned char smem[];
-  axis0_body<float, N, DIM>(a, smem, blockIdx.x, gridDim.x);
+  axis0_body<float, N, E, DIM>(a, smem, blockIdx.x, gridDim.x);
 }
 
 // ---- last axis: input stage, r2c ---------------------------------------------
@@ -307,10 +326,10 @@ struct LArgs {
   const c_t<T>* tw;
 };
 
-template <typename T, int N, int DIM, int IN>
+template <typename T, int N, int E, int DIM, int IN>
 __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
-  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
-  constexpr int NT = L * fp(N), NH = N / 2 + 1;
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = lstride(L);
+  constexpr int NT = L * (N / E), NH = N / 2 + 1;
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
   const int64_t row0 = bx * RB;
@@ -352,11 +371,11 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   __syncthreads();
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
-  c_t<T> v[fe(N)];
-  load_first<T, N>(v, p, [&](int m) { return at(m); });
+  c_t<T> v[E];
+  load_first<T, N, E>(v, p, [&](int m) { return at(m); });
   __syncthreads();
-  Passes<T, N, -1, 0, 1>::run(v, p, a.tw, at);
-  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  Passes<T, N, E, -1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
   __syncthreads();
   const int64_t obase = row0 * NH * NC, olim = a.rows * NH * NC;
   for (int e = threadIdx.x; e < RB * NH * NC; e += NT) {
@@ -378,16 +397,16 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   }
 }
 
-template <typename T, int N, int DIM, int IN>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_fast(LArgs<T> a) {
+template <typename T, int N, int E, int DIM, int IN>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_r2c_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  r2c_body<T, N, DIM, IN>(a, smem, blockIdx.x);
+  r2c_body<T, N, E, DIM, IN>(a, smem, blockIdx.x);
 }
 
 // batched: the first iteration of a solve takes p as it is, later ones
 // update it (p = -r + beta p) on the way in
-template <int N, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_batch(
+template <int N, int E, int DIM>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_r2c_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -402,8 +421,8 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_
   a.rows = rows;
   a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
-  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, DIM, IN_TAN>(a, smem, blockIdx.x);
-  else r2c_body<float, N, DIM, IN_TAN_PUPDATE>(a, smem, blockIdx.x);
+  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN>(a, smem, blockIdx.x);
+  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE>(a, smem, blockIdx.x);
 }
 
 // ---- last axis: c2r, output stage ---------------------------------------------
@@ -411,10 +430,10 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_r2c_
 // symmetry (imaginary parts of k = 0 and k = n/2 discarded) and combined into
 // Z = A + i B, whose backward transform holds row 2 rp in its real and row
 // 2 rp + 1 in its imaginary part.
-template <typename T, int N, int DIM, int OUT>
+template <typename T, int N, int E, int DIM, int OUT>
 __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
-  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = L + 1;
-  constexpr int NT = L * fp(N), NH = N / 2 + 1;
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = lstride(L);
+  constexpr int NT = L * (N / E), NH = N / 2 + 1;
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int64_t row0 = bx * RB;
   {
@@ -448,8 +467,8 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
   __syncthreads();
   const int l = threadIdx.x % L, p = threadIdx.x / L;
   const int rp = l / NC, cc = l - rp * NC;
-  c_t<T> v[fe(N)];
-  load_first<T, N>(v, p, [&](int i) {
+  c_t<T> v[E];
+  load_first<T, N, E>(v, p, [&](int i) {
     const int kk = i <= N / 2 ? i : N - i;
     c_t<T> A = s[((2 * rp) * NH + kk) * NC + cc];
     c_t<T> B = s[((2 * rp + 1) * NH + kk) * NC + cc];
@@ -468,8 +487,8 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
   });
   __syncthreads();
   auto at = [&](int m) -> c_t<T>& { return s[m * LS + l]; };
-  Passes<T, N, +1, 0, 1>::run(v, p, a.tw, at);
-  store_last<T, N>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
+  Passes<T, N, E, +1, 0, 1>::run(v, p, a.tw, at);
+  store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
   __syncthreads();
   const T* sr = reinterpret_cast<const T*>(smem);
   const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
@@ -506,14 +525,14 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
   }
 }
 
-template <typename T, int N, int DIM, int OUT>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_fast(LArgs<T> a) {
+template <typename T, int N, int E, int DIM, int OUT>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_c2r_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  c2r_body<T, N, DIM, OUT>(a, smem, blockIdx.x);
+  c2r_body<T, N, E, DIM, OUT>(a, smem, blockIdx.x);
 }
 
-template <int N, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_batch(
+template <int N, int E, int DIM>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_c2r_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -525,7 +544,7 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * fp(N)) void k_c2r_
   a.rows = rows;
   a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
-  c2r_body<float, N, DIM, OUT_DOT>(a, smem, blockIdx.x);
+  c2r_body<float, N, E, DIM, OUT_DOT>(a, smem, blockIdx.x);
 }
 
 // ---- launchers ----------------------------------------------------------------
@@ -536,10 +555,10 @@ const c_t<T>* tw_of(const FftPlan& p, int d) {
 
 template <int N, int DIM>
 constexpr size_t lines_lds(size_t csz) {
-  return (size_t)N * (z_rows(N, DIM) / 2 * DIM * DIM + 1) * csz;
+  return (size_t)N * lstride(z_rows(N, DIM) / 2 * DIM * DIM) * csz;
 }
 
-template <typename T, int N, int DIM>
+template <typename T, int N, int E, int DIM>
 int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   LArgs<T> a{};
   a.in = reinterpret_cast<const T*>(op.in);
@@ -551,19 +570,19 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.spec = reinterpret_cast<c_t<T>*>(spec);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
-  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * fp(N);
+  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
   const size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
   const dim3 grid((unsigned)((a.rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)));
   switch (op.in_mode) {
-    case IN_PLAIN: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_PLAIN>), grid, dim3(NT), lds, stream(), a); break;
-    case IN_TAN: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_TAN>), grid, dim3(NT), lds, stream(), a); break;
-    default: hipLaunchKernelGGL((k_r2c_fast<T, N, DIM, IN_TAN_PUPDATE>), grid, dim3(NT), lds, stream(), a); break;
+    case IN_PLAIN: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN>), grid, dim3(NT), lds, stream(), a); break;
+    case IN_TAN: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN>), grid, dim3(NT), lds, stream(), a); break;
+    default: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE>), grid, dim3(NT), lds, stream(), a); break;
   }
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <typename T, int N, int DIM>
+template <typename T, int N, int E, int DIM>
 int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   LArgs<T> a{};
   a.spec = reinterpret_cast<c_t<T>*>(spec);
@@ -573,40 +592,40 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
   constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
-  constexpr int NT = RB / 2 * NC * fp(N);
+  constexpr int NT = RB / 2 * NC * (N / E);
   const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(c_t<T>);
   size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
   if (tile > lds) lds = tile;
   if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   if (op.out_mode == OUT_DOT)
-    hipLaunchKernelGGL((k_c2r_fast<T, N, DIM, OUT_DOT>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_DOT>), grid, dim3(NT), lds, stream(), a);
   else
-    hipLaunchKernelGGL((k_c2r_fast<T, N, DIM, OUT_PLAIN>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_PLAIN>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <typename T, int N>
+template <typename T, int N, int E>
 int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   int64_t inner = p.nc;
   for (int d = ax + 1; d < p.dim; ++d) inner *= p.hres[d];
   int64_t outer = 1;
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
-  constexpr int L = s_lines<T>();
-  constexpr int NT = L * fp(N);
-  const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
+  constexpr int L = s_lines<T, N, E>();
+  constexpr int NT = L * (N / E);
+  const size_t lds = (size_t)N * lstride(L) * sizeof(c_t<T>);
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
   c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
   if (sign < 0)
-    hipLaunchKernelGGL((k_strided_fast<T, N, -1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+    hipLaunchKernelGGL((k_strided_fast<T, N, E, -1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
   else
-    hipLaunchKernelGGL((k_strided_fast<T, N, +1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+    hipLaunchKernelGGL((k_strided_fast<T, N, E, +1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <typename T, int N, int DIM>
+template <typename T, int N, int E, int DIM>
 int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   XArgs<T> a{};
   a.data = reinterpret_cast<c_t<T>*>(spec);
@@ -620,117 +639,127 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   }
   a.hlast = p.hres[DIM - 1];
   a.norm = (T)pj.norm;
-  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * fp(N);
-  const size_t lds = (size_t)N * (L + 1) * sizeof(c_t<T>);
+  constexpr int KP = x_kpts(N, DIM, sizeof(T) == 4, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
+  const size_t lds = (size_t)N * lstride(L) * sizeof(c_t<T>);
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
-  hipLaunchKernelGGL((k_axis0_fast<T, N, DIM>), grid, dim3(NT), lds, stream(), a);
+  hipLaunchKernelGGL((k_axis0_fast<T, N, E, DIM>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-// dispatch on the axis length
-#define MUGPU_FAST_SWITCH(n, CALL) \
-  switch (n) {                     \
-    case 16: return CALL(16);      \
-    case 32: return CALL(32);      \
-    case 48: return CALL(48);      \
-    case 64: return CALL(64);      \
-    case 96: return CALL(96);      \
-    case 128: return CALL(128);    \
-    case 256: return CALL(256);    \
-    case 512: return CALL(512);    \
-    default: break;                \
+// dispatch on the axis length and the variant (0: E = 16, or 12 for the
+// lengths with a factor 3; 1: E = 8, powers of two only)
+#define MUGPU_FAST_SWITCH(n, v, CALL)                  \
+  switch (n) {                                         \
+    case 16: return (v) ? CALL(16, 8) : CALL(16, 16);  \
+    case 32: return (v) ? CALL(32, 8) : CALL(32, 16);  \
+    case 48: return CALL(48, 12);                      \
+    case 64: return (v) ? CALL(64, 8) : CALL(64, 16);  \
+    case 96: return CALL(96, 12);                      \
+    case 128: return (v) ? CALL(128, 8) : CALL(128, 16); \
+    case 256: return (v) ? CALL(256, 8) : CALL(256, 16); \
+    case 512: return (v) ? CALL(512, 8) : CALL(512, 16); \
+    default: break;                                    \
   }
+
+template <typename T>
+constexpr int pi_of() {
+  return sizeof(T) == 8 ? 0 : 1;
+}
 
 template <typename T, int DIM>
 int r2c_any(FftPlan& p, const OpSpec& op, void* spec) {
-#define C_(n) r2c_n<T, n, DIM>(p, op, spec)
-  MUGPU_FAST_SWITCH(p.res[DIM - 1], C_)
+  const int v = p.fvar[pi_of<T>()][FV_R2C][op.tan_kind == TAN_ISO ? 1 : 0];
+#define C_(n, e) r2c_n<T, n, e, DIM>(p, op, spec)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], v, C_)
 #undef C_
   return MUGPU_EARG;
 }
 template <typename T, int DIM>
 int c2r_any(FftPlan& p, const OpSpec& op, void* spec) {
-#define C_(n) c2r_n<T, n, DIM>(p, op, spec)
-  MUGPU_FAST_SWITCH(p.res[DIM - 1], C_)
+  const int v = p.fvar[pi_of<T>()][FV_C2R][0];
+#define C_(n, e) c2r_n<T, n, e, DIM>(p, op, spec)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], v, C_)
 #undef C_
   return MUGPU_EARG;
 }
 template <typename T>
 int strided_any(FftPlan& p, int ax, void* spec, int sign) {
-#define C_(n) strided_n<T, n>(p, ax, spec, sign)
-  MUGPU_FAST_SWITCH(p.res[ax], C_)
+  const int v = p.fvar[pi_of<T>()][ax == 0 ? FV_STRIDED0 : FV_STRIDED1][0];
+#define C_(n, e) strided_n<T, n, e>(p, ax, spec, sign)
+  MUGPU_FAST_SWITCH(p.res[ax], v, C_)
 #undef C_
   return MUGPU_EARG;
 }
 template <typename T, int DIM>
 int axis0_any(FftPlan& p, const ProjParams& pj, void* spec) {
-#define C_(n) axis0_n<T, n, DIM>(p, pj, spec)
-  MUGPU_FAST_SWITCH(p.res[0], C_)
+  const int v = p.fvar[pi_of<T>()][FV_AXIS0][0];
+#define C_(n, e) axis0_n<T, n, e, DIM>(p, pj, spec)
+  MUGPU_FAST_SWITCH(p.res[0], v, C_)
 #undef C_
   return MUGPU_EARG;
 }
 
 // ---- batched launchers (FP32, slots on blockIdx.z) ------------------------------
-template <int N, int DIM>
+template <int N, int E, int DIM>
 int b_r2c(FftPlan& p, const BatchSlot* slots, int ns) {
-  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * fp(N);
+  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
   const int64_t rows = p.N / N;
   const size_t lds = lines_lds<N, DIM>(sizeof(float2));
   const dim3 grid((unsigned)((rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_r2c_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+  hipLaunchKernelGGL((k_r2c_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <int N, int DIM>
+template <int N, int E, int DIM>
 int b_c2r(FftPlan& p, const BatchSlot* slots, int ns) {
   constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
-  constexpr int NT = RB / 2 * NC * fp(N);
+  constexpr int NT = RB / 2 * NC * (N / E);
   const int64_t rows = p.N / N;
   const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(float2);
   size_t lds = lines_lds<N, DIM>(sizeof(float2));
   if (tile > lds) lds = tile;
   if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
   const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_c2r_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+  hipLaunchKernelGGL((k_c2r_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <int N>
+template <int N, int E>
 int b_strided(FftPlan& p, int ax, int sign, const BatchSlot* slots, int ns) {
   int64_t inner = p.nc;
   for (int d = ax + 1; d < p.dim; ++d) inner *= p.hres[d];
   int64_t outer = 1;
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
-  constexpr int L = s_lines<float>();
-  constexpr int NT = L * fp(N);
-  const size_t lds = (size_t)N * (L + 1) * sizeof(float2);
+  constexpr int L = s_lines<float, N, E>();
+  constexpr int NT = L * (N / E);
+  const size_t lds = (size_t)N * lstride(L) * sizeof(float2);
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer, (unsigned)ns);
   if (sign < 0)
-    hipLaunchKernelGGL((k_strided_batch<N, -1>), grid, dim3(NT), lds, stream(), slots, inner,
+    hipLaunchKernelGGL((k_strided_batch<N, E, -1>), grid, dim3(NT), lds, stream(), slots, inner,
                        tw_of<float>(p, ax));
   else
-    hipLaunchKernelGGL((k_strided_batch<N, +1>), grid, dim3(NT), lds, stream(), slots, inner,
+    hipLaunchKernelGGL((k_strided_batch<N, E, +1>), grid, dim3(NT), lds, stream(), slots, inner,
                        tw_of<float>(p, ax));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <int N, int DIM>
+template <int N, int E, int DIM>
 int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
   XArgs<float> a{};
   a.plane = p.kpts / p.res[0];
   a.tw = tw_of<float>(p, 0);
   for (int d = 0; d < 3; ++d) a.res[d] = p.res[d];
   a.hlast = p.hres[DIM - 1];
-  constexpr int KP = x_kpts(N, DIM), NC = DIM * DIM, L = KP * NC, NT = L * fp(N);
-  const size_t lds = (size_t)N * (L + 1) * sizeof(float2);
+  constexpr int KP = x_kpts(N, DIM, true, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
+  const size_t lds = (size_t)N * lstride(L) * sizeof(float2);
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_axis0_batch<N, DIM>), grid, dim3(NT), lds, stream(), slots, a);
+  hipLaunchKernelGGL((k_axis0_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -738,37 +767,38 @@ int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
 template <int DIM>
 int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
   int rc = MUGPU_EARG;
+  const unsigned char(&fv)[FV_COUNT][2] = p.fvar[1];
   timing_begin(MUGPU_K_LAST_R2C);
-#define C_(n) b_r2c<n, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_r2c<n, e, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_R2C][1], C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_R2C);
   if (rc) return rc;
   if (DIM == 3) {
     timing_begin(MUGPU_K_MIDDLE);
-#define C_(n) b_strided<n>(p, 1, -1, slots, ns)
-    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_strided<n, e>(p, 1, -1, slots, ns)
+    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], fv[FV_STRIDED1][0], C_) return MUGPU_EARG; }();
 #undef C_
     timing_end(MUGPU_K_MIDDLE);
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_AXIS0);
-#define C_(n) b_axis0<n, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_axis0<n, e, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], fv[FV_AXIS0][0], C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_AXIS0);
   if (rc) return rc;
   if (DIM == 3) {
     timing_begin(MUGPU_K_MIDDLE);
-#define C_(n) b_strided<n>(p, 1, +1, slots, ns)
-    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_strided<n, e>(p, 1, +1, slots, ns)
+    rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[1], fv[FV_STRIDED1][0], C_) return MUGPU_EARG; }();
 #undef C_
     timing_end(MUGPU_K_MIDDLE);
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_LAST_C2R);
-#define C_(n) b_c2r<n, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_c2r<n, e, DIM>(p, slots, ns)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_C2R][0], C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_C2R);
   return rc;
@@ -777,6 +807,23 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
 }  // namespace
 
 int fast_rows_per_block(int n, int dim) { return z_rows(n, dim); }
+
+// variants chosen by the autotuner (muspectre_amd/autotune.py writes this
+// table from timings of every variant on the GPU); lengths with a factor 3
+// have one variant
+namespace {
+struct FastTuned {
+  int pass, n, dim, prec, iso, variant;
+};
+#include "fft_tuned.inc"
+}  // namespace
+
+int fast_variant(int pass, int n, int dim, int prec, int iso) {
+  if (n % 3 == 0) return 0;
+  for (const FastTuned& t : kFastTuned)
+    if (t.pass == pass && t.n == n && t.dim == dim && t.prec == prec && t.iso == iso) return t.variant;
+  return 0;
+}
 
 int fast_batch_operator(FftPlan& p, const BatchSlot* d_slots, int nslots) {
   for (int d = 0; d < p.dim; ++d)

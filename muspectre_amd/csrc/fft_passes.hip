@@ -458,6 +458,21 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
            (size_t)lines * odd_stride(len) * 16 <= lds;
   };
   for (int d = 0; d < dim; ++d) p.fastax[d] = fast_size(res[d]);
+  // variants of the compile-time-length passes, from the tuned table
+  for (int pr = 0; pr < 2; ++pr)
+    for (int iso = 0; iso < 2; ++iso) {
+      const int prec = pr == 0 ? MUGPU_FP64 : MUGPU_FP32;
+      if (p.fastax[dim - 1]) {
+        p.fvar[pr][FV_R2C][iso] = (unsigned char)fast_variant(FV_R2C, res[dim - 1], dim, prec, iso);
+        p.fvar[pr][FV_C2R][iso] = (unsigned char)fast_variant(FV_C2R, res[dim - 1], dim, prec, 0);
+      }
+      if (p.fastax[0]) {
+        p.fvar[pr][FV_AXIS0][iso] = (unsigned char)fast_variant(FV_AXIS0, res[0], dim, prec, 0);
+        p.fvar[pr][FV_STRIDED0][iso] = (unsigned char)fast_variant(FV_STRIDED0, res[0], dim, prec, 0);
+      }
+      if (dim == 3 && p.fastax[1])
+        p.fvar[pr][FV_STRIDED1][iso] = (unsigned char)fast_variant(FV_STRIDED1, res[1], dim, prec, 0);
+    }
   const int n = res[dim - 1];
   const int64_t rows = p.N / n;
   if (p.fastax[dim - 1]) {

@@ -94,7 +94,17 @@ struct FftPlan {
   void* partials = nullptr;  // double per z-pass workgroup
   int64_t nzblocks = 0;
   bool fastax[3] = {false, false, false};  // compile-time-length passes (fft_fast.hip)
+  // variant of each compile-time-length pass (0: 16 or 12 values per thread,
+  // 1: 8 values per thread), per precision (0 FP64, 1 FP32), per pass and,
+  // for the r2c pass, per tangent (0 full matrices, 1 isotropic)
+  unsigned char fvar[2][5][2] = {};
 };
+// passes with variants
+enum FastVariantPass { FV_R2C = 0, FV_STRIDED1 = 1, FV_AXIS0 = 2, FV_C2R = 3, FV_STRIDED0 = 4, FV_COUNT = 5 };
+// variant of a pass for an axis length, dimension, precision and tangent
+// class, from the tuned table or an override (fft_tune.cc semantics: see
+// mugpu_fft_set_variant)
+int fast_variant(int pass, int n, int dim, int prec, int iso);
 
 int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec);
 void plan_free(FftPlan& p);
