@@ -185,6 +185,37 @@ constexpr int s_lines() {
   return 256 / (int)sizeof(c_t<T>) * (N / E) <= 1024 ? 256 / (int)sizeof(c_t<T>) : 1024 / (N / E);
 }
 
+// ---- LDS per workgroup and occupancy ----------------------------------------
+template <typename T, int N, int DIM>
+constexpr size_t lds_lines() {
+  return (size_t)N * lstride(z_rows(N, DIM) / 2 * DIM * DIM) * sizeof(c_t<T>);
+}
+template <typename T, int N, int E, int DIM>
+constexpr size_t lds_c2r() {
+  constexpr size_t lines = lds_lines<T, N, DIM>();
+  constexpr size_t tile = (size_t)z_rows(N, DIM) * (N / 2 + 1) * DIM * DIM * sizeof(c_t<T>);
+  constexpr size_t red = (size_t)z_rows(N, DIM) / 2 * DIM * DIM * (N / E) * sizeof(double);
+  return lines > tile ? (lines > red ? lines : red) : (tile > red ? tile : red);
+}
+template <typename T, int N, int E>
+constexpr size_t lds_strided() {
+  return (size_t)N * lstride(s_lines<T, N, E>()) * sizeof(c_t<T>);
+}
+template <typename T, int N, int E, int DIM>
+constexpr size_t lds_axis0() {
+  return (size_t)N * lstride(x_kpts(N, DIM, sizeof(T) == 4, E) * DIM * DIM) * sizeof(c_t<T>);
+}
+// Waves per SIMD the register allocation has to leave room for: two
+// workgroups per CU wherever the 160 KiB of LDS hold two (one otherwise).
+// Without the bound the compiler may take enough registers per lane that a
+// second workgroup no longer fits, and a pass that alternates HBM phases
+// with LDS phases then leaves HBM idle during the latter.
+constexpr size_t kLdsPerCu = 160 * 1024;
+__host__ __device__ constexpr int wpe(int threads, size_t lds) {
+  return ((lds * 2 <= kLdsPerCu ? 2 : 1) * ((threads + 63) / 64) + 3) / 4;
+}
+#define MUGPU_WPE(threads, lds) __attribute__((amdgpu_waves_per_eu(wpe((threads), (lds)))))
+
 // ---- strided c2c (middle axis, or first axis without projection) ----------
 // data[(o * N + m) * inner + w]: workgroup (w-tile, o) transforms lines
 // w0 .. w0 + L - 1 of plane o
@@ -207,7 +238,8 @@ __device__ __forceinline__ void strided_body(c_t<T>* __restrict__ data, int64_t 
 }
 
 template <typename T, int N, int E, int SG>
-__global__ __launch_bounds__((s_lines<T, N, E>() * (N / E))) void k_strided_fast(
+__global__ __launch_bounds__((s_lines<T, N, E>() * (N / E)))
+MUGPU_WPE((s_lines<T, N, E>() * (N / E)), (lds_strided<T, N, E>())) void k_strided_fast(
     c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem[];
   strided_body<T, N, E, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
@@ -215,7 +247,8 @@ __global__ __launch_bounds__((s_lines<T, N, E>() * (N / E))) void k_strided_fast
 
 // batched (FP32, one RVE per blockIdx.z)
 template <int N, int E, int SG>
-__global__ __launch_bounds__((s_lines<float, N, E>() * (N / E))) void k_strided_batch(
+__global__ __launch_bounds__((s_lines<float, N, E>() * (N / E)))
+MUGPU_WPE((s_lines<float, N, E>() * (N / E)), (lds_strided<float, N, E>())) void k_strided_batch(
     const BatchSlot* __restrict__ slots, int64_t inner, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -286,6 +319,8 @@ __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* sme
 }
 
 template <typename T, int N, int E, int DIM>
+// (no register bound: the projection stage needs its registers, and spilling
+// them costs more than the second workgroup per CU gains)
 __global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   axis0_body<T, N, E, DIM>(a, smem, blockIdx.x, gridDim.x);
@@ -326,21 +361,82 @@ struct LArgs {
   const c_t<T>* tw;
 };
 
-template <typename T, int N, int E, int DIM, int IN>
+// ISO: the tangent is known to be isotropic (TAN_ISO), which the compiler
+// folds in; otherwise a.tan_kind selects it at run time
+template <typename T, int N, int E, int DIM, int IN, bool ISO>
 __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
   constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = lstride(L);
   constexpr int NT = L * (N / E), NH = N / 2 + 1;
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
   const int64_t row0 = bx * RB;
-  if (IN == IN_PLAIN) {
+  if constexpr (IN == IN_PLAIN) {
     const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
     for (int e = threadIdx.x; e < RB * N * NC; e += NT) {
       const int r = e / (N * NC), rem = e - r * (N * NC);
       const int m = rem / NC, c = rem - m * NC;
       sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = base + e < lim ? a.in[base + e] : T(0);
     }
+  } else if constexpr (sizeof(T) == 4 || ISO) {
+    // p (updated on the way in: p = -r + beta p) is staged in LDS by an
+    // element-wise sweep, consecutive lanes on consecutive values, so that
+    // the AoS fields are read and written in full cache lines; clamped
+    // unconditional loads, kBatch per thread in flight.  (FP64 with a large
+    // tangent takes the per-voxel form below: there the workgroups are few
+    // per CU, and the tangent loads of a voxel are better issued together
+    // with its p and r loads than after a barrier.)
+    const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
+    {
+      constexpr int TOT = RB * N * NC, kBatch = 8;
+      const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
+      for (int e0 = 0; e0 < TOT; e0 += kBatch * NT) {
+        T pv[kBatch], rv[kBatch];
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) {
+          const int e = e0 + i * NT + threadIdx.x;
+          const int64_t g = base + e < lim ? base + e : lim - 1;
+          pv[i] = a.in[g];
+          if (IN == IN_TAN_PUPDATE) rv[i] = a.rvec[g];
+        }
+#pragma unroll
+        for (int i = 0; i < kBatch; ++i) {
+          const int e = e0 + i * NT + threadIdx.x;
+          if (e < TOT) {
+            const bool valid = base + e < lim;
+            T x = T(0);
+            if (valid) {
+              if (IN == IN_TAN_PUPDATE) {
+                x = -rv[i] + beta * pv[i];
+                a.pout[base + e] = x;
+              } else {
+                x = pv[i];
+              }
+            }
+            const int r = e / (N * NC), rem = e - r * (N * NC);
+            const int m = rem / NC, c = rem - m * NC;
+            sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = x;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // K:p per voxel, in place: each voxel's components are read and written
+    // by one thread
+    for (int e = threadIdx.x; e < RB * N; e += NT) {
+      const int r = e / N, m = e - (e / N) * N;
+      const int64_t row = row0 + r;
+      if (row < a.rows) {
+        T pv[NC], dp[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pv[c] = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
+        tangent_product<T, DIM>(ISO ? TAN_ISO : a.tan_kind, a.tan, row * N + m, pv, dp);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
+      }
+    }
   } else {
+    // one voxel per thread and step: p, r and tangent loads issued together
+    // (FP64, tangent kind chosen at run time)
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
     for (int e = threadIdx.x; e < RB * N; e += NT) {
       const int r = e / N, m = e - (e / N) * N;
@@ -359,7 +455,7 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
 #pragma unroll
           for (int c = 0; c < NC; ++c) pv[c] = a.in[vox * NC + c];
         }
-        tangent_product<T, DIM>(a.tan_kind, a.tan, vox, pv, dp);
+        tangent_product<T, DIM>(ISO ? TAN_ISO : a.tan_kind, a.tan, vox, pv, dp);
       } else {
 #pragma unroll
         for (int c = 0; c < NC; ++c) dp[c] = T(0);
@@ -397,16 +493,19 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   }
 }
 
-template <typename T, int N, int E, int DIM, int IN>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_r2c_fast(LArgs<T> a) {
+template <typename T, int N, int E, int DIM, int IN, bool ISO>
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM>())) void k_r2c_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  r2c_body<T, N, E, DIM, IN>(a, smem, blockIdx.x);
+  r2c_body<T, N, E, DIM, IN, ISO>(a, smem, blockIdx.x);
 }
 
 // batched: the first iteration of a solve takes p as it is, later ones
-// update it (p = -r + beta p) on the way in
+// update it (p = -r + beta p) on the way in; the served material is
+// isotropic (TAN_ISO tangents)
 template <int N, int E, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_r2c_batch(
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM>())) void k_r2c_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -421,8 +520,8 @@ __global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_r2
   a.rows = rows;
   a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
-  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN>(a, smem, blockIdx.x);
-  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE>(a, smem, blockIdx.x);
+  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN, true>(a, smem, blockIdx.x);
+  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE, true>(a, smem, blockIdx.x);
 }
 
 // ---- last axis: c2r, output stage ---------------------------------------------
@@ -526,13 +625,15 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
 }
 
 template <typename T, int N, int E, int DIM, int OUT>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_c2r_fast(LArgs<T> a) {
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_c2r<T, N, E, DIM>())) void k_c2r_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   c2r_body<T, N, E, DIM, OUT>(a, smem, blockIdx.x);
 }
 
 template <int N, int E, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E)) void k_c2r_batch(
+__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_c2r<float, N, E, DIM>())) void k_c2r_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -553,11 +654,6 @@ const c_t<T>* tw_of(const FftPlan& p, int d) {
   return reinterpret_cast<const c_t<T>*>(sizeof(T) == 8 ? p.tw64[d] : p.tw32[d]);
 }
 
-template <int N, int DIM>
-constexpr size_t lines_lds(size_t csz) {
-  return (size_t)N * lstride(z_rows(N, DIM) / 2 * DIM * DIM) * csz;
-}
-
 template <typename T, int N, int E, int DIM>
 int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   LArgs<T> a{};
@@ -571,13 +667,19 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
   constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
-  const size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
+  constexpr size_t lds = lds_lines<T, N, DIM>();
   const dim3 grid((unsigned)((a.rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)));
-  switch (op.in_mode) {
-    case IN_PLAIN: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN>), grid, dim3(NT), lds, stream(), a); break;
-    case IN_TAN: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN>), grid, dim3(NT), lds, stream(), a); break;
-    default: hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE>), grid, dim3(NT), lds, stream(), a); break;
-  }
+  const bool iso = op.tan_kind == TAN_ISO;
+  if (op.in_mode == IN_PLAIN)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN, false>), grid, dim3(NT), lds, stream(), a);
+  else if (op.in_mode == IN_TAN && iso)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, true>), grid, dim3(NT), lds, stream(), a);
+  else if (op.in_mode == IN_TAN)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, false>), grid, dim3(NT), lds, stream(), a);
+  else if (iso)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, true>), grid, dim3(NT), lds, stream(), a);
+  else
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, false>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -593,10 +695,7 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.tw = tw_of<T>(p, DIM - 1);
   constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
-  const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(c_t<T>);
-  size_t lds = lines_lds<N, DIM>(sizeof(c_t<T>));
-  if (tile > lds) lds = tile;
-  if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
+  constexpr size_t lds = lds_c2r<T, N, E, DIM>();
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   if (op.out_mode == OUT_DOT)
     hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_DOT>), grid, dim3(NT), lds, stream(), a);
@@ -614,7 +713,7 @@ int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   constexpr int L = s_lines<T, N, E>();
   constexpr int NT = L * (N / E);
-  const size_t lds = (size_t)N * lstride(L) * sizeof(c_t<T>);
+  constexpr size_t lds = lds_strided<T, N, E>();
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
   c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
   if (sign < 0)
@@ -640,7 +739,7 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   a.hlast = p.hres[DIM - 1];
   a.norm = (T)pj.norm;
   constexpr int KP = x_kpts(N, DIM, sizeof(T) == 4, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  const size_t lds = (size_t)N * lstride(L) * sizeof(c_t<T>);
+  constexpr size_t lds = lds_axis0<T, N, E, DIM>();
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
   hipLaunchKernelGGL((k_axis0_fast<T, N, E, DIM>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
@@ -705,7 +804,7 @@ template <int N, int E, int DIM>
 int b_r2c(FftPlan& p, const BatchSlot* slots, int ns) {
   constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
   const int64_t rows = p.N / N;
-  const size_t lds = lines_lds<N, DIM>(sizeof(float2));
+  constexpr size_t lds = lds_lines<float, N, DIM>();
   const dim3 grid((unsigned)((rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_r2c_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
@@ -718,10 +817,7 @@ int b_c2r(FftPlan& p, const BatchSlot* slots, int ns) {
   constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
   const int64_t rows = p.N / N;
-  const size_t tile = (size_t)RB * (N / 2 + 1) * NC * sizeof(float2);
-  size_t lds = lines_lds<N, DIM>(sizeof(float2));
-  if (tile > lds) lds = tile;
-  if ((size_t)NT * sizeof(double) > lds) lds = (size_t)NT * sizeof(double);
+  constexpr size_t lds = lds_c2r<float, N, E, DIM>();
   const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_c2r_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
@@ -737,7 +833,7 @@ int b_strided(FftPlan& p, int ax, int sign, const BatchSlot* slots, int ns) {
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   constexpr int L = s_lines<float, N, E>();
   constexpr int NT = L * (N / E);
-  const size_t lds = (size_t)N * lstride(L) * sizeof(float2);
+  constexpr size_t lds = lds_strided<float, N, E>();
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer, (unsigned)ns);
   if (sign < 0)
     hipLaunchKernelGGL((k_strided_batch<N, E, -1>), grid, dim3(NT), lds, stream(), slots, inner,
@@ -757,7 +853,7 @@ int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
   for (int d = 0; d < 3; ++d) a.res[d] = p.res[d];
   a.hlast = p.hres[DIM - 1];
   constexpr int KP = x_kpts(N, DIM, true, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  const size_t lds = (size_t)N * lstride(L) * sizeof(float2);
+  constexpr size_t lds = lds_axis0<float, N, E, DIM>();
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_axis0_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, a);
   MUGPU_HIP(hipGetLastError());
