@@ -313,8 +313,10 @@ int mugpu_timing_start(int kernel_class);
 int mugpu_timing_stop(int64_t *launches, double *total_ms);
 int mugpu_timing_class(int kernel_class, int64_t *launches, double *total_ms);
 /* autotuner measurement: average duration in ms of one compile-time-length
- * FFT pass (MUGPU_PASS_*) in one variant (0: 16 or 12 values per thread,
- * 1: 8 values per thread) over `reps` launches on zeroed scratch buffers of
+ * FFT pass (MUGPU_PASS_*) in one variant (bit 0: 8 values per thread instead
+ * of 16 or 12; bit 1, MUGPU_PASS_AXIS0 only: half the k-points per
+ * workgroup; MUGPU_EARG for a variant that does not exist at this length)
+ * over `reps` launches on zeroed scratch buffers of
  * the grid `resolutions`; iso selects the isotropic tangent for the r2c
  * pass.  The chosen variants are compiled into the library
  * (muspectre_amd/csrc/fft_tuned.inc, written by muspectre_amd/autotune.py). */

@@ -4,14 +4,16 @@ Every pass of the operator (last-axis r2c with the p update and K:p,
 middle-axis c2c, first-axis c2c with the projection, last-axis c2r with the
 p.Ap partials, first-axis c2c of the engine transforms) exists in two
 variants for the power-of-two lengths: 16 complex values per thread, or 8
-with twice the threads on the same tile.  Which one is faster depends on
-the pass, the length, the dimension and the precision.  This script times
-both variants of every pass on the GPU (mugpu_fft_tune_pass, zeroed scratch
-buffers of a grid large enough to fill the device) and writes the table the
-library is compiled with (csrc/fft_tuned.inc).  A variant replaces the
-default only when it is faster by more than the margin, so the table does
-not flip on timing noise; the table is part of the source, so results are
-the same from run to run.
+with twice the threads on the same tile (bit 0 of the variant).  The
+first-axis projection pass also comes with half the k-points per workgroup
+(bit 1): half the LDS, more workgroups per CU, shorter HBM runs.  Which one
+is fastest depends on the pass, the length, the dimension and the
+precision.  This script times every variant of every pass on the GPU
+(mugpu_fft_tune_pass, zeroed scratch buffers of a grid large enough to fill
+the device) and writes the table the library is compiled with
+(csrc/fft_tuned.inc).  A variant replaces the default only when it is
+faster by more than the margin, so the table does not flip on timing noise;
+the table is part of the source, so results are the same from run to run.
 
     python -m muspectre_amd.autotune --out muspectre_amd/csrc/fft_tuned.inc \\
         --summary tuning.md
@@ -67,12 +69,22 @@ def main():
                 for iso in ((0, 1) if pass_ == 0 else (0,)):
                     for n in LENGTHS:
                         res = grid(pass_, n, dim)
-                        t = [time_pass(lib, res, prec, pass_, v, iso, args.reps) for v in (0, 1)]
-                        best = 1 if (t[1] < (1.0 - args.margin) * t[0]
-                                     and t[0] - t[1] > args.min_gain_ms) else 0
+                        t = {}
+                        for v in ((0, 1, 2, 3) if pass_ == 2 else (0, 1)):
+                            try:
+                                t[v] = time_pass(lib, res, prec, pass_, v, iso, args.reps)
+                            except _capi.MugpuError:
+                                pass  # no such variant at this length
+                        best = 0
+                        for v, tv in t.items():
+                            if (v and tv < (1.0 - args.margin) * t[0] and t[0] - tv > args.min_gain_ms
+                                    and tv < t[best]):
+                                best = v
                         rows.append((dim, pname, PASSES[pass_], iso, n, res, t, best))
-                        print("dim {} {} {:26s} iso {} n {:4d} grid {}: {:.3f} / {:.3f} ms -> {}".format(
-                            dim, pname, PASSES[pass_], iso, n, res, t[0], t[1], best), flush=True)
+                        print("dim {} {} {:26s} iso {} n {:4d} grid {}: {} ms -> {}".format(
+                            dim, pname, PASSES[pass_], iso, n, res,
+                            " / ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(4)),
+                            best), flush=True)
                         if best:
                             entries.append((pass_, n, dim, prec, iso, best))
     lines = ["// Variants of the compile-time-length FFT passes: {pass, axis length, dim,",
@@ -94,12 +106,16 @@ def main():
                     "with a factor 3) only when it is faster by more than {:g} % and {:g} us. The "
                     "table is compiled into libmugpu.so (muspectre_amd/csrc/fft_tuned.inc).\n\n".format(
                         args.reps, 100 * args.margin, 1000 * args.min_gain_ms))
-            f.write("| dim | precision | pass | isotropic tangent | n | grid | ms, 16 (12) per thread | "
-                    "ms, 8 per thread | chosen |\n|---|---|---|---|---|---|---|---|---|\n")
+            f.write("Variants: 0 = 16 (12) values per thread, 1 = 8 values per thread, "
+                    "2 and 3 = the same with half the k-points per workgroup (first-axis "
+                    "projection pass only); - = no such variant at this length.\n\n")
+            f.write("| dim | precision | pass | isotropic tangent | n | grid | ms, variant 0 | "
+                    "variant 1 | variant 2 | variant 3 | chosen |\n"
+                    "|---|---|---|---|---|---|---|---|---|---|---|\n")
             for dim, pname, pn, iso, n, res, t, best in rows:
-                f.write("| {} | {} | {} | {} | {} | {} | {:.3f} | {:.3f} | {} |\n".format(
-                    dim, pname, pn, iso, n, "x".join(map(str, res)), t[0], t[1],
-                    "8" if best else "16"))
+                f.write("| {} | {} | {} | {} | {} | {} | {} | {} |\n".format(
+                    dim, pname, pn, iso, n, "x".join(map(str, res)),
+                    " | ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(4)), best))
     print("wrote {} ({} entries)".format(os.path.abspath(args.out), len(entries)))
 
 

@@ -343,10 +343,11 @@ extern "C" int mugpu_timing_class(int kernel_class, int64_t* launches, double* t
 extern "C" int mugpu_fft_tune_pass(int dim, const int* res, int precision, int pass, int variant,
                                    int iso, int reps, double* avg_ms) {
   MUGPU_TRY(ensure_init());
-  if (!res || !avg_ms || reps < 1 || pass < 0 || pass >= FV_COUNT || variant < 0 || variant > 1)
+  if (!res || !avg_ms || reps < 1 || pass < 0 || pass >= FV_COUNT || dim < 2 || dim > 3)
     return MUGPU_EARG;
   const int axis = (pass == FV_R2C || pass == FV_C2R) ? dim - 1 : pass == FV_STRIDED1 ? 1 : 0;
-  if ((pass == FV_STRIDED1 && dim != 3) || !fast_size(res[axis]) || (variant && res[axis] % 3 == 0)) {
+  if ((pass == FV_STRIDED1 && dim != 3) || !fast_size(res[axis]) ||
+      !fast_variant_valid(pass, res[axis], dim, precision, variant)) {
     set_error("no variant %d of pass %d for an axis of %d points", variant, pass, res[axis]);
     return MUGPU_EARG;
   }

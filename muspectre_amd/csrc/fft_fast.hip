@@ -169,11 +169,16 @@ __host__ __device__ constexpr int z_rows(int n, int dim) {
 }
 // first-axis projection pass: k-points per workgroup (same thread counts in
 // FP64; FP32 takes twice as many for the same LDS and longer HBM runs)
-__host__ __device__ constexpr int x_kpts(int n, int dim, bool f32, int e) {
+__host__ __device__ constexpr int x_kpts0(int n, int dim, bool f32, int e) {
   // at most 1024 threads per workgroup
   return (dim == 3 ? 32 / ft(n) : 64 / ft(n)) * (f32 ? 2 : 1) * dim * dim * (n / e) <= 1024
              ? (dim == 3 ? 32 / ft(n) : 64 / ft(n)) * (f32 ? 2 : 1)
              : (dim == 3 ? 32 / ft(n) : 64 / ft(n));
+}
+// variant with half the k-points (h = 1): half the LDS per workgroup, so
+// more workgroups per CU, for shorter HBM runs
+__host__ __device__ constexpr int x_kpts(int n, int dim, bool f32, int e, int h) {
+  return h && x_kpts0(n, dim, f32, e) >= 2 ? x_kpts0(n, dim, f32, e) / 2 : x_kpts0(n, dim, f32, e);
 }
 // LDS line stride: the line count rounded up to an odd number (lanes on
 // consecutive lines then spread over the banks, and no padding is spent when
@@ -201,9 +206,9 @@ template <typename T, int N, int E>
 constexpr size_t lds_strided() {
   return (size_t)N * lstride(s_lines<T, N, E>()) * sizeof(c_t<T>);
 }
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 constexpr size_t lds_axis0() {
-  return (size_t)N * lstride(x_kpts(N, DIM, sizeof(T) == 4, E) * DIM * DIM) * sizeof(c_t<T>);
+  return (size_t)N * lstride(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM) * sizeof(c_t<T>);
 }
 // Waves per SIMD the register allocation has to leave room for: two
 // workgroups per CU wherever the 160 KiB of LDS hold two (one otherwise).
@@ -271,10 +276,10 @@ struct XArgs {
   T norm;
 };
 
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* smem, int64_t bx,
                                            int64_t gx) {
-  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM, sizeof(T) == 4, E), L = KP * NC, LS = lstride(L),
+  constexpr int NC = DIM * DIM, KP = x_kpts(N, DIM, sizeof(T) == 4, E, H), L = KP * NC, LS = lstride(L),
                 NT = L * (N / E);
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int l = threadIdx.x % L, p = threadIdx.x / L;
@@ -318,16 +323,16 @@ __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* sme
   if (valid) store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { base[m * inner] = x; });
 }
 
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 // (no register bound: the projection stage needs its registers, and spilling
 // them costs more than the second workgroup per CU gains)
-__global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
+__global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  axis0_body<T, N, E, DIM>(a, smem, blockIdx.x, gridDim.x);
+  axis0_body<T, N, E, DIM, H>(a, smem, blockIdx.x, gridDim.x);
 }
 
-template <int N, int E, int DIM>
-__global__ __launch_bounds__(x_kpts(N, DIM, true, E) * DIM * DIM * (N / E)) void k_axis0_batch(
+template <int N, int E, int DIM, int H>
+__global__ __launch_bounds__(x_kpts(N, DIM, true, E, H) * DIM * DIM * (N / E)) void k_axis0_batch(
     const BatchSlot* __restrict__ slots, XArgs<float> shape) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -338,7 +343,7 @@ __global__ __launch_bounds__(x_kpts(N, DIM, true, E) * DIM * DIM * (N / E)) void
   for (int d = 0; d < 3; ++d) a.L[d] = sl.L[d];
   a.norm = (float)sl.norm;
   extern __shared__ __align__(16) unsigned char smem[];
-  axis0_body<float, N, E, DIM>(a, smem, blockIdx.x, gridDim.x);
+  axis0_body<float, N, E, DIM, H>(a, smem, blockIdx.x, gridDim.x);
 }
 
 // ---- last axis: input stage, r2c ---------------------------------------------
@@ -724,7 +729,7 @@ int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   return MUGPU_OK;
 }
 
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   XArgs<T> a{};
   a.data = reinterpret_cast<c_t<T>*>(spec);
@@ -738,10 +743,10 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   }
   a.hlast = p.hres[DIM - 1];
   a.norm = (T)pj.norm;
-  constexpr int KP = x_kpts(N, DIM, sizeof(T) == 4, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  constexpr size_t lds = lds_axis0<T, N, E, DIM>();
+  constexpr int KP = x_kpts(N, DIM, sizeof(T) == 4, E, H), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
+  constexpr size_t lds = lds_axis0<T, N, E, DIM, H>();
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
-  hipLaunchKernelGGL((k_axis0_fast<T, N, E, DIM>), grid, dim3(NT), lds, stream(), a);
+  hipLaunchKernelGGL((k_axis0_fast<T, N, E, DIM, H>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -790,11 +795,20 @@ int strided_any(FftPlan& p, int ax, void* spec, int sign) {
 #undef C_
   return MUGPU_EARG;
 }
+// first-axis pass: bit 0 of the variant selects E = 8, bit 1 half the
+// k-points per workgroup; a variant that changes nothing at this length
+// runs the kernel it equals
+template <typename T, int N, int E, int DIM>
+int axis0_h(FftPlan& p, const ProjParams& pj, void* spec, int h) {
+  if constexpr (x_kpts(N, DIM, sizeof(T) == 4, E, 1) != x_kpts(N, DIM, sizeof(T) == 4, E, 0))
+    if (h) return axis0_n<T, N, E, DIM, 1>(p, pj, spec);
+  return axis0_n<T, N, E, DIM, 0>(p, pj, spec);
+}
 template <typename T, int DIM>
 int axis0_any(FftPlan& p, const ProjParams& pj, void* spec) {
   const int v = p.fvar[pi_of<T>()][FV_AXIS0][0];
-#define C_(n, e) axis0_n<T, n, e, DIM>(p, pj, spec)
-  MUGPU_FAST_SWITCH(p.res[0], v, C_)
+#define C_(n, e) axis0_h<T, n, e, DIM>(p, pj, spec, v >> 1)
+  MUGPU_FAST_SWITCH(p.res[0], v & 1, C_)
 #undef C_
   return MUGPU_EARG;
 }
@@ -845,19 +859,26 @@ int b_strided(FftPlan& p, int ax, int sign, const BatchSlot* slots, int ns) {
   return MUGPU_OK;
 }
 
-template <int N, int E, int DIM>
+template <int N, int E, int DIM, int H>
 int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
   XArgs<float> a{};
   a.plane = p.kpts / p.res[0];
   a.tw = tw_of<float>(p, 0);
   for (int d = 0; d < 3; ++d) a.res[d] = p.res[d];
   a.hlast = p.hres[DIM - 1];
-  constexpr int KP = x_kpts(N, DIM, true, E), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  constexpr size_t lds = lds_axis0<float, N, E, DIM>();
+  constexpr int KP = x_kpts(N, DIM, true, E, H), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
+  constexpr size_t lds = lds_axis0<float, N, E, DIM, H>();
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_axis0_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, a);
+  hipLaunchKernelGGL((k_axis0_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
+}
+
+template <int N, int E, int DIM>
+int b_axis0_h(FftPlan& p, const BatchSlot* slots, int ns, int h) {
+  if constexpr (x_kpts(N, DIM, true, E, 1) != x_kpts(N, DIM, true, E, 0))
+    if (h) return b_axis0<N, E, DIM, 1>(p, slots, ns);
+  return b_axis0<N, E, DIM, 0>(p, slots, ns);
 }
 
 template <int DIM>
@@ -879,8 +900,8 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_AXIS0);
-#define C_(n, e) b_axis0<n, e, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], fv[FV_AXIS0][0], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_axis0_h<n, e, DIM>(p, slots, ns, fv[FV_AXIS0][0] >> 1)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], fv[FV_AXIS0][0] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_AXIS0);
   if (rc) return rc;
@@ -915,10 +936,22 @@ struct FastTuned {
 }  // namespace
 
 int fast_variant(int pass, int n, int dim, int prec, int iso) {
-  if (n % 3 == 0) return 0;
   for (const FastTuned& t : kFastTuned)
-    if (t.pass == pass && t.n == n && t.dim == dim && t.prec == prec && t.iso == iso) return t.variant;
+    if (t.pass == pass && t.n == n && t.dim == dim && t.prec == prec && t.iso == iso &&
+        fast_variant_valid(pass, n, dim, prec, t.variant))
+      return t.variant;
   return 0;
+}
+
+bool fast_variant_valid(int pass, int n, int dim, int prec, int variant) {
+  if (variant < 0 || variant > (pass == FV_AXIS0 ? 3 : 1)) return false;
+  if ((variant & 1) && n % 3 == 0) return false;
+  if (variant & 2) {
+    const int e = (variant & 1) ? 8 : fe_default(n);
+    const bool f32 = prec != MUGPU_FP64;
+    return x_kpts(n, dim, f32, e, 1) != x_kpts(n, dim, f32, e, 0);
+  }
+  return true;
 }
 
 int fast_batch_operator(FftPlan& p, const BatchSlot* d_slots, int nslots) {

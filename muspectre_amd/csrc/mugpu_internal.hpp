@@ -105,6 +105,9 @@ enum FastVariantPass { FV_R2C = 0, FV_STRIDED1 = 1, FV_AXIS0 = 2, FV_C2R = 3, FV
 // class, from the tuned table or an override (fft_tune.cc semantics: see
 // mugpu_fft_set_variant)
 int fast_variant(int pass, int n, int dim, int prec, int iso);
+// whether a pass variant exists (and differs from variant 0's tile) for
+// this length, dimension and precision
+bool fast_variant_valid(int pass, int n, int dim, int prec, int variant);
 
 int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec);
 void plan_free(FftPlan& p);
