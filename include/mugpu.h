@@ -314,9 +314,11 @@ int mugpu_timing_stop(int64_t *launches, double *total_ms);
 int mugpu_timing_class(int kernel_class, int64_t *launches, double *total_ms);
 /* autotuner measurement: average duration in ms of one compile-time-length
  * FFT pass (MUGPU_PASS_*) in one variant (bit 0: 8 values per thread instead
- * of 16 or 12; bit 1, MUGPU_PASS_AXIS0 only: half the k-points per
- * workgroup; MUGPU_EARG for a variant that does not exist at this length)
- * over `reps` launches on zeroed scratch buffers of
+ * of 16 or 12; bit 1, for MUGPU_PASS_R2C, MUGPU_PASS_C2R and
+ * MUGPU_PASS_AXIS0: half the rows or k-points per workgroup; MUGPU_EARG for
+ * a variant that does not exist at this length)
+ * over `reps` launches, each timed alone after a copy that evicts the
+ * last-level cache, on zeroed scratch buffers of
  * the grid `resolutions`; iso selects the isotropic tangent for the r2c
  * pass.  The chosen variants are compiled into the library
  * (muspectre_amd/csrc/fft_tuned.inc, written by muspectre_amd/autotune.py). */

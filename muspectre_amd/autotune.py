@@ -5,8 +5,9 @@ middle-axis c2c, first-axis c2c with the projection, last-axis c2r with the
 p.Ap partials, first-axis c2c of the engine transforms) exists in two
 variants for the power-of-two lengths: 16 complex values per thread, or 8
 with twice the threads on the same tile (bit 0 of the variant).  The
-first-axis projection pass also comes with half the k-points per workgroup
-(bit 1): half the LDS, more workgroups per CU, shorter HBM runs.  Which one
+last-axis passes and the first-axis projection pass also come with half the
+rows or k-points per workgroup (bit 1): half the LDS, more workgroups per CU
+(and, on the first axis, shorter HBM runs).  Which one
 is fastest depends on the pass, the length, the dimension and the
 precision.  This script times every variant of every pass on the GPU
 (mugpu_fft_tune_pass, zeroed scratch buffers of a grid large enough to fill
@@ -30,10 +31,12 @@ LENGTHS = [16, 32, 64, 128, 256, 512]
 
 
 def grid(pass_, n, dim):
-    """A grid with the tuned axis of length n and the other axes large enough
-    to fill the device."""
+    """A grid with the tuned axis of length n: the cube n^3 for n = 128 and
+    256 (the large single-RVE grids), otherwise other axes of 256 points, large
+    enough to fill the device (short axes stand for many small RVEs served in
+    one batched launch; 512^3 behaves like 256 x 256 x 512)."""
     if dim == 3:
-        m = 256
+        m = n if n in (128, 256) else 256
         return {0: [m, m, n], 3: [m, m, n], 1: [m, n, m], 2: [n, m, m], 4: [n, m, m]}[pass_]
     m = 512  # the longest outer axis with compiled passes
     return {0: [m, n], 3: [m, n], 2: [n, m], 4: [n, m]}[pass_]
@@ -70,7 +73,7 @@ def main():
                     for n in LENGTHS:
                         res = grid(pass_, n, dim)
                         t = {}
-                        for v in ((0, 1, 2, 3) if pass_ == 2 else (0, 1)):
+                        for v in ((0, 1, 2, 3) if pass_ in (0, 2, 3) else (0, 1)):
                             try:
                                 t[v] = time_pass(lib, res, prec, pass_, v, iso, args.reps)
                             except _capi.MugpuError:
@@ -101,14 +104,16 @@ def main():
             f.write("# FFT pass variants chosen by the autotuner\n\nCommand (on one MI355X):\n\n"
                     "    python3 -u -m muspectre_amd.autotune --out muspectre_amd/csrc/fft_tuned.inc "
                     "--summary profiles/fft_autotune.md\n\n"
-                    "Average of {} launches per variant (HIP events), zeroed scratch buffers; a "
+                    "Average of {} launches per variant (HIP events around each launch, after "
+                    "a 1 GiB copy that evicts the last-level cache), zeroed scratch buffers; a "
                     "variant with 8 values per thread replaces the default (16, or 12 for lengths "
                     "with a factor 3) only when it is faster by more than {:g} % and {:g} us. The "
                     "table is compiled into libmugpu.so (muspectre_amd/csrc/fft_tuned.inc).\n\n".format(
                         args.reps, 100 * args.margin, 1000 * args.min_gain_ms))
             f.write("Variants: 0 = 16 (12) values per thread, 1 = 8 values per thread, "
-                    "2 and 3 = the same with half the k-points per workgroup (first-axis "
-                    "projection pass only); - = no such variant at this length.\n\n")
+                    "2 and 3 = the same with half the rows (last-axis passes) or k-points "
+                    "(first-axis projection pass) per workgroup; - = no such variant at this "
+                    "length.\n\n")
             f.write("| dim | precision | pass | isotropic tangent | n | grid | ms, variant 0 | "
                     "variant 1 | variant 2 | variant 3 | chosen |\n"
                     "|---|---|---|---|---|---|---|---|---|---|---|\n")

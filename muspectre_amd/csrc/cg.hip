@@ -222,7 +222,7 @@ int batch_cg_update(const BatchSlot* d_slots, int nslots, int64_t n) {
 int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots) {
   const int64_t n = p.N * p.nc;
   MUGPU_TRY(fast_batch_operator(p, d_slots, nslots));
-  MUGPU_TRY(batch_finalize(d_slots, nslots, p.nzblocks, FIN_ALPHA));
+  MUGPU_TRY(batch_finalize(d_slots, nslots, p.nzb[1], FIN_ALPHA));
   MUGPU_TRY(batch_cg_update(d_slots, nslots, n));
   return batch_finalize(d_slots, nslots, grid_for(n), FIN_BETA);
 }

@@ -163,9 +163,13 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t total) {
 
 // ---- shapes of the workgroups -----------------------------------------------
 // last-axis passes: rows per workgroup (even: two rows share a complex
-// line); 288 threads in 3D, 256 in 2D
-__host__ __device__ constexpr int z_rows(int n, int dim) {
+// line); 288 threads in 3D, 256 in 2D; h = 1: half the rows (half the LDS,
+// more workgroups per CU) where that leaves an even count
+__host__ __device__ constexpr int z_rows0(int n, int dim) {
   return dim == 3 ? 64 / ft(n) : 128 / ft(n);
+}
+__host__ __device__ constexpr int z_rows(int n, int dim, int h) {
+  return h && z_rows0(n, dim) >= 4 ? z_rows0(n, dim) / 2 : z_rows0(n, dim);
 }
 // first-axis projection pass: k-points per workgroup (same thread counts in
 // FP64; FP32 takes twice as many for the same LDS and longer HBM runs)
@@ -191,15 +195,15 @@ constexpr int s_lines() {
 }
 
 // ---- LDS per workgroup and occupancy ----------------------------------------
-template <typename T, int N, int DIM>
+template <typename T, int N, int DIM, int H>
 constexpr size_t lds_lines() {
-  return (size_t)N * lstride(z_rows(N, DIM) / 2 * DIM * DIM) * sizeof(c_t<T>);
+  return (size_t)N * lstride(z_rows(N, DIM, H) / 2 * DIM * DIM) * sizeof(c_t<T>);
 }
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 constexpr size_t lds_c2r() {
-  constexpr size_t lines = lds_lines<T, N, DIM>();
-  constexpr size_t tile = (size_t)z_rows(N, DIM) * (N / 2 + 1) * DIM * DIM * sizeof(c_t<T>);
-  constexpr size_t red = (size_t)z_rows(N, DIM) / 2 * DIM * DIM * (N / E) * sizeof(double);
+  constexpr size_t lines = lds_lines<T, N, DIM, H>();
+  constexpr size_t tile = (size_t)z_rows(N, DIM, H) * (N / 2 + 1) * DIM * DIM * sizeof(c_t<T>);
+  constexpr size_t red = (size_t)z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E) * sizeof(double);
   return lines > tile ? (lines > red ? lines : red) : (tile > red ? tile : red);
 }
 template <typename T, int N, int E>
@@ -368,9 +372,9 @@ struct LArgs {
 
 // ISO: the tangent is known to be isotropic (TAN_ISO), which the compiler
 // folds in; otherwise a.tan_kind selects it at run time
-template <typename T, int N, int E, int DIM, int IN, bool ISO>
+template <typename T, int N, int E, int DIM, int IN, bool ISO, int H>
 __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
-  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = lstride(L);
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM, H), L = RB / 2 * NC, LS = lstride(L);
   constexpr int NT = L * (N / E), NH = N / 2 + 1;
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
@@ -498,19 +502,19 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   }
 }
 
-template <typename T, int N, int E, int DIM, int IN, bool ISO>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
-MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM>())) void k_r2c_fast(LArgs<T> a) {
+template <typename T, int N, int E, int DIM, int IN, bool ISO, int H>
+__global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM, H>())) void k_r2c_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  r2c_body<T, N, E, DIM, IN, ISO>(a, smem, blockIdx.x);
+  r2c_body<T, N, E, DIM, IN, ISO, H>(a, smem, blockIdx.x);
 }
 
 // batched: the first iteration of a solve takes p as it is, later ones
 // update it (p = -r + beta p) on the way in; the served material is
 // isotropic (TAN_ISO tangents)
-template <int N, int E, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
-MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM>())) void k_r2c_batch(
+template <int N, int E, int DIM, int H>
+__global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM, H>())) void k_r2c_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -525,8 +529,8 @@ MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM>())
   a.rows = rows;
   a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
-  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN, true>(a, smem, blockIdx.x);
-  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE, true>(a, smem, blockIdx.x);
+  if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN, true, H>(a, smem, blockIdx.x);
+  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE, true, H>(a, smem, blockIdx.x);
 }
 
 // ---- last axis: c2r, output stage ---------------------------------------------
@@ -534,9 +538,9 @@ MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM>())
 // symmetry (imaginary parts of k = 0 and k = n/2 discarded) and combined into
 // Z = A + i B, whose backward transform holds row 2 rp in its real and row
 // 2 rp + 1 in its imaginary part.
-template <typename T, int N, int E, int DIM, int OUT>
+template <typename T, int N, int E, int DIM, int OUT, int H>
 __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem, int64_t bx) {
-  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM), L = RB / 2 * NC, LS = lstride(L);
+  constexpr int NC = DIM * DIM, RB = z_rows(N, DIM, H), L = RB / 2 * NC, LS = lstride(L);
   constexpr int NT = L * (N / E), NH = N / 2 + 1;
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int64_t row0 = bx * RB;
@@ -629,16 +633,16 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
   }
 }
 
-template <typename T, int N, int E, int DIM, int OUT>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
-MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_c2r<T, N, E, DIM>())) void k_c2r_fast(LArgs<T> a) {
+template <typename T, int N, int E, int DIM, int OUT, int H>
+__global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_c2r<T, N, E, DIM, H>())) void k_c2r_fast(LArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  c2r_body<T, N, E, DIM, OUT>(a, smem, blockIdx.x);
+  c2r_body<T, N, E, DIM, OUT, H>(a, smem, blockIdx.x);
 }
 
-template <int N, int E, int DIM>
-__global__ __launch_bounds__(z_rows(N, DIM) / 2 * DIM * DIM * (N / E))
-MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_c2r<float, N, E, DIM>())) void k_c2r_batch(
+template <int N, int E, int DIM, int H>
+__global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
+MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_c2r<float, N, E, DIM, H>())) void k_c2r_batch(
     const BatchSlot* __restrict__ slots, int64_t rows, const float2* __restrict__ tw) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
@@ -650,7 +654,7 @@ MUGPU_WPE(z_rows(N, DIM) / 2 * DIM * DIM * (N / E), (lds_c2r<float, N, E, DIM>()
   a.rows = rows;
   a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
-  c2r_body<float, N, E, DIM, OUT_DOT>(a, smem, blockIdx.x);
+  c2r_body<float, N, E, DIM, OUT_DOT, H>(a, smem, blockIdx.x);
 }
 
 // ---- launchers ----------------------------------------------------------------
@@ -659,7 +663,7 @@ const c_t<T>* tw_of(const FftPlan& p, int d) {
   return reinterpret_cast<const c_t<T>*>(sizeof(T) == 8 ? p.tw64[d] : p.tw32[d]);
 }
 
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   LArgs<T> a{};
   a.in = reinterpret_cast<const T*>(op.in);
@@ -671,25 +675,26 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.spec = reinterpret_cast<c_t<T>*>(spec);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
-  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
-  constexpr size_t lds = lds_lines<T, N, DIM>();
-  const dim3 grid((unsigned)((a.rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)));
+  constexpr int RB = z_rows(N, DIM, H);
+  constexpr int NT = RB / 2 * DIM * DIM * (N / E);
+  constexpr size_t lds = lds_lines<T, N, DIM, H>();
+  const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   const bool iso = op.tan_kind == TAN_ISO;
   if (op.in_mode == IN_PLAIN)
-    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN, false>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN, false, H>), grid, dim3(NT), lds, stream(), a);
   else if (op.in_mode == IN_TAN && iso)
-    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, true>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, true, H>), grid, dim3(NT), lds, stream(), a);
   else if (op.in_mode == IN_TAN)
-    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, false>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, false, H>), grid, dim3(NT), lds, stream(), a);
   else if (iso)
-    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, true>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, true, H>), grid, dim3(NT), lds, stream(), a);
   else
-    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, false>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, false, H>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <typename T, int N, int E, int DIM>
+template <typename T, int N, int E, int DIM, int H>
 int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   LArgs<T> a{};
   a.spec = reinterpret_cast<c_t<T>*>(spec);
@@ -698,14 +703,14 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.partials = reinterpret_cast<double*>(p.partials);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
-  constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
+  constexpr int RB = z_rows(N, DIM, H), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
-  constexpr size_t lds = lds_c2r<T, N, E, DIM>();
+  constexpr size_t lds = lds_c2r<T, N, E, DIM, H>();
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   if (op.out_mode == OUT_DOT)
-    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_DOT>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_DOT, H>), grid, dim3(NT), lds, stream(), a);
   else
-    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_PLAIN>), grid, dim3(NT), lds, stream(), a);
+    hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_PLAIN, H>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
@@ -771,19 +776,34 @@ constexpr int pi_of() {
   return sizeof(T) == 8 ? 0 : 1;
 }
 
+// last-axis passes: bit 0 of the variant selects E = 8, bit 1 half the rows
+// per workgroup; a variant that changes nothing at this length runs the
+// kernel it equals
+template <typename T, int N, int E, int DIM>
+int r2c_h(FftPlan& p, const OpSpec& op, void* spec, int h) {
+  if constexpr (z_rows(N, DIM, 1) != z_rows(N, DIM, 0))
+    if (h) return r2c_n<T, N, E, DIM, 1>(p, op, spec);
+  return r2c_n<T, N, E, DIM, 0>(p, op, spec);
+}
+template <typename T, int N, int E, int DIM>
+int c2r_h(FftPlan& p, const OpSpec& op, void* spec, int h) {
+  if constexpr (z_rows(N, DIM, 1) != z_rows(N, DIM, 0))
+    if (h) return c2r_n<T, N, E, DIM, 1>(p, op, spec);
+  return c2r_n<T, N, E, DIM, 0>(p, op, spec);
+}
 template <typename T, int DIM>
 int r2c_any(FftPlan& p, const OpSpec& op, void* spec) {
   const int v = p.fvar[pi_of<T>()][FV_R2C][op.tan_kind == TAN_ISO ? 1 : 0];
-#define C_(n, e) r2c_n<T, n, e, DIM>(p, op, spec)
-  MUGPU_FAST_SWITCH(p.res[DIM - 1], v, C_)
+#define C_(n, e) r2c_h<T, n, e, DIM>(p, op, spec, v >> 1)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], v & 1, C_)
 #undef C_
   return MUGPU_EARG;
 }
 template <typename T, int DIM>
 int c2r_any(FftPlan& p, const OpSpec& op, void* spec) {
   const int v = p.fvar[pi_of<T>()][FV_C2R][0];
-#define C_(n, e) c2r_n<T, n, e, DIM>(p, op, spec)
-  MUGPU_FAST_SWITCH(p.res[DIM - 1], v, C_)
+#define C_(n, e) c2r_h<T, n, e, DIM>(p, op, spec, v >> 1)
+  MUGPU_FAST_SWITCH(p.res[DIM - 1], v & 1, C_)
 #undef C_
   return MUGPU_EARG;
 }
@@ -814,26 +834,27 @@ int axis0_any(FftPlan& p, const ProjParams& pj, void* spec) {
 }
 
 // ---- batched launchers (FP32, slots on blockIdx.z) ------------------------------
-template <int N, int E, int DIM>
+template <int N, int E, int DIM, int H>
 int b_r2c(FftPlan& p, const BatchSlot* slots, int ns) {
-  constexpr int NT = z_rows(N, DIM) / 2 * DIM * DIM * (N / E);
+  constexpr int RB = z_rows(N, DIM, H);
+  constexpr int NT = RB / 2 * DIM * DIM * (N / E);
   const int64_t rows = p.N / N;
-  constexpr size_t lds = lds_lines<float, N, DIM>();
-  const dim3 grid((unsigned)((rows + z_rows(N, DIM) - 1) / z_rows(N, DIM)), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_r2c_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+  constexpr size_t lds = lds_lines<float, N, DIM, H>();
+  const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
+  hipLaunchKernelGGL((k_r2c_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
-template <int N, int E, int DIM>
+template <int N, int E, int DIM, int H>
 int b_c2r(FftPlan& p, const BatchSlot* slots, int ns) {
-  constexpr int RB = z_rows(N, DIM), NC = DIM * DIM;
+  constexpr int RB = z_rows(N, DIM, H), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
   const int64_t rows = p.N / N;
-  constexpr size_t lds = lds_c2r<float, N, E, DIM>();
+  constexpr size_t lds = lds_c2r<float, N, E, DIM, H>();
   const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
-  hipLaunchKernelGGL((k_c2r_batch<N, E, DIM>), grid, dim3(NT), lds, stream(), slots, rows,
+  hipLaunchKernelGGL((k_c2r_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
@@ -875,6 +896,18 @@ int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
 }
 
 template <int N, int E, int DIM>
+int b_r2c_h(FftPlan& p, const BatchSlot* slots, int ns, int h) {
+  if constexpr (z_rows(N, DIM, 1) != z_rows(N, DIM, 0))
+    if (h) return b_r2c<N, E, DIM, 1>(p, slots, ns);
+  return b_r2c<N, E, DIM, 0>(p, slots, ns);
+}
+template <int N, int E, int DIM>
+int b_c2r_h(FftPlan& p, const BatchSlot* slots, int ns, int h) {
+  if constexpr (z_rows(N, DIM, 1) != z_rows(N, DIM, 0))
+    if (h) return b_c2r<N, E, DIM, 1>(p, slots, ns);
+  return b_c2r<N, E, DIM, 0>(p, slots, ns);
+}
+template <int N, int E, int DIM>
 int b_axis0_h(FftPlan& p, const BatchSlot* slots, int ns, int h) {
   if constexpr (x_kpts(N, DIM, true, E, 1) != x_kpts(N, DIM, true, E, 0))
     if (h) return b_axis0<N, E, DIM, 1>(p, slots, ns);
@@ -886,8 +919,8 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
   int rc = MUGPU_EARG;
   const unsigned char(&fv)[FV_COUNT][2] = p.fvar[1];
   timing_begin(MUGPU_K_LAST_R2C);
-#define C_(n, e) b_r2c<n, e, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_R2C][1], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_r2c_h<n, e, DIM>(p, slots, ns, fv[FV_R2C][1] >> 1)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_R2C][1] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_R2C);
   if (rc) return rc;
@@ -914,8 +947,8 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_LAST_C2R);
-#define C_(n, e) b_c2r<n, e, DIM>(p, slots, ns)
-  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_C2R][0], C_) return MUGPU_EARG; }();
+#define C_(n, e) b_c2r_h<n, e, DIM>(p, slots, ns, fv[FV_C2R][0] >> 1)
+  rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_C2R][0] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_C2R);
   return rc;
@@ -923,7 +956,7 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
 
 }  // namespace
 
-int fast_rows_per_block(int n, int dim) { return z_rows(n, dim); }
+int fast_rows_per_block(int n, int dim, int variant) { return z_rows(n, dim, (variant >> 1) & 1); }
 
 // variants chosen by the autotuner (muspectre_amd/autotune.py writes this
 // table from timings of every variant on the GPU); lengths with a factor 3
@@ -944,9 +977,11 @@ int fast_variant(int pass, int n, int dim, int prec, int iso) {
 }
 
 bool fast_variant_valid(int pass, int n, int dim, int prec, int variant) {
-  if (variant < 0 || variant > (pass == FV_AXIS0 ? 3 : 1)) return false;
+  const bool halves = pass == FV_AXIS0 || pass == FV_R2C || pass == FV_C2R;
+  if (variant < 0 || variant > (halves ? 3 : 1)) return false;
   if ((variant & 1) && n % 3 == 0) return false;
   if (variant & 2) {
+    if (pass != FV_AXIS0) return z_rows(n, dim, 1) != z_rows(n, dim, 0);
     const int e = (variant & 1) ? 8 : fe_default(n);
     const bool f32 = prec != MUGPU_FP64;
     return x_kpts(n, dim, f32, e, 1) != x_kpts(n, dim, f32, e, 0);

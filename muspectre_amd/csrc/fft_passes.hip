@@ -476,7 +476,11 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
   const int n = res[dim - 1];
   const int64_t rows = p.N / n;
   if (p.fastax[dim - 1]) {
-    p.zrb = fast_rows_per_block(n, dim);
+    for (int pr = 0; pr < 2; ++pr) {
+      const int rb = fast_rows_per_block(n, dim, p.fvar[pr][FV_C2R][0]);
+      p.nzb[pr] = (rows + rb - 1) / rb;
+    }
+    p.zrb = fast_rows_per_block(n, dim, 0);
   } else {
     if (!fits((nc + 1) / 2, n, kNT)) {
       set_error("last axis of %d points exceeds the supported line length", n);
@@ -488,7 +492,8 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
       ++rb;
     p.zrb = rb;
   }
-  p.nzblocks = (rows + p.zrb - 1) / p.zrb;
+  if (!p.fastax[dim - 1]) p.nzb[0] = p.nzb[1] = (rows + p.zrb - 1) / p.zrb;
+  p.nzblocks = p.nzb[0] > p.nzb[1] ? p.nzb[0] : p.nzb[1];
   // strided middle axis: up to 16 lines (256 B rows in FP64)
   if (dim == 3 && !p.fastax[1]) {
     const int ny = res[1];
@@ -523,7 +528,14 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     p.xNT = NT;
     p.xWk = Wk;
   }
-  MUGPU_HIP(hipMalloc(&p.partials, sizeof(double) * (size_t)(p.nzblocks > kRedBlocks ? p.nzblocks : kRedBlocks)));
+  // room for the c2r partials of every variant (the autotuner times them all
+  // on one plan): the half-rows variant launches the most workgroups
+  int64_t nzmax = p.nzblocks;
+  if (p.fastax[dim - 1]) {
+    const int rbmin = fast_rows_per_block(res[dim - 1], dim, 2);
+    nzmax = (p.N / res[dim - 1] + rbmin - 1) / rbmin;
+  }
+  MUGPU_HIP(hipMalloc(&p.partials, sizeof(double) * (size_t)(nzmax > kRedBlocks ? nzmax : kRedBlocks)));
   return MUGPU_OK;
 }
 

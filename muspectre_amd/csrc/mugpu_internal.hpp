@@ -92,18 +92,20 @@ struct FftPlan {
   int xWk = 1;     // fused axis-0 projection pass: k-points per workgroup
   int xNT = 256;   // threads of the fused pass
   void* partials = nullptr;  // double per z-pass workgroup
-  int64_t nzblocks = 0;
+  int64_t nzblocks = 0;      // z-pass workgroups, the largest of nzb
+  int64_t nzb[2] = {0, 0};   // c2r workgroups (p.Ap partials) per precision (0 FP64, 1 FP32)
   bool fastax[3] = {false, false, false};  // compile-time-length passes (fft_fast.hip)
-  // variant of each compile-time-length pass (0: 16 or 12 values per thread,
-  // 1: 8 values per thread), per precision (0 FP64, 1 FP32), per pass and,
-  // for the r2c pass, per tangent (0 full matrices, 1 isotropic)
+  // variant of each compile-time-length pass (bit 0: 8 values per thread
+  // instead of 16 or 12; bit 1: half the rows or k-points per workgroup, for
+  // the last-axis and first-axis projection passes), per precision (0 FP64,
+  // 1 FP32), per pass and, for the r2c pass, per tangent (0 full matrices,
+  // 1 isotropic)
   unsigned char fvar[2][5][2] = {};
 };
 // passes with variants
 enum FastVariantPass { FV_R2C = 0, FV_STRIDED1 = 1, FV_AXIS0 = 2, FV_C2R = 3, FV_STRIDED0 = 4, FV_COUNT = 5 };
 // variant of a pass for an axis length, dimension, precision and tangent
-// class, from the tuned table or an override (fft_tune.cc semantics: see
-// mugpu_fft_set_variant)
+// class, from the tuned table (fft_tuned.inc)
 int fast_variant(int pass, int n, int dim, int prec, int iso);
 // whether a pass variant exists (and differs from variant 0's tile) for
 // this length, dimension and precision
@@ -149,7 +151,8 @@ int run_c2r(FftPlan& p, int prec, void* spectrum, void* out);
 struct BatchSlot;
 // passes for axis lengths 16, 32, 48, 64, 96, 128, 256, 512 (fft_fast.hip)
 bool fast_size(int n);
-int fast_rows_per_block(int n, int dim);
+// rows per workgroup of the last-axis passes in a variant
+int fast_rows_per_block(int n, int dim, int variant);
 int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_c2r(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_strided(FftPlan& p, int prec, int ax, void* spec, int sign);

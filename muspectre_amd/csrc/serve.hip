@@ -150,8 +150,10 @@ Layout layout_of(int dim, const int* res) {
   const int nc = dim * dim;
   const int64_t nd = N * nc;
   const int64_t rows = N / res[dim - 1];
-  const int64_t nz = (rows + fast_rows_per_block(res[dim - 1], dim) - 1) /
-                     fast_rows_per_block(res[dim - 1], dim);
+  // c2r workgroups of the FP32 operator (one p.Ap partial each)
+  const int rb = fast_rows_per_block(
+      res[dim - 1], dim, fast_variant(FV_C2R, res[dim - 1], dim, MUGPU_FP32, 0));
+  const int64_t nz = (rows + rb - 1) / rb;
   Layout l{};
   size_t o = 0;
   auto take = [&](size_t bytes) {
