@@ -205,18 +205,31 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
 
 /* ---- SolverCG (src/solver/solver_cg.cc:54-110) -----------------------
  * Device-resident CG.  The solver owns x, r, p, Ap (solver_cg.cc:45-51);
- * the host loop keeps the reference's control flow and reads back one
- * scalar per iteration.
+ * the host loop keeps the reference's control flow and reads back r.r once
+ * per iteration (mugpu_cg_step) or once per chunk of iterations
+ * (mugpu_cg_steps).
  * mugpu_cg_start: x = 0; r = A x - b (one full operator application);
  *   p = -r; returns rdr = r.r and rhs_norm2 = b.b.
  * mugpu_cg_step: [if update_p: p = -r + beta p with beta from the previous
  *   step] Ap = A p; alpha = rdr/(p.Ap); x += alpha p; r += alpha Ap;
  *   new_rdr = r.r; beta = new_rdr/rdr.  Returns new_rdr.
- * In FP32 cell precision the operator runs in FP32 on the FP64 vectors. */
+ * mugpu_cg_steps: up to max_steps iterations of mugpu_cg_step (update_p
+ *   applies to the first; the later ones update p) queued at once, with
+ *   the loop test of solver_cg.cc:70-96 applied on the device after each
+ *   iteration: the next one runs while fewer than max_steps have run and
+ *   new_rdr > tol2 (the first always runs).  *steps returns the iterations
+ *   run (one whose r.r is NaN counts and stops the loop), *new_rdr the r.r
+ *   of the last.  The host reads back once per call instead of once per
+ *   iteration; the arithmetic is that of mugpu_cg_step.
+ * The operator runs in FP64 (cells created with MUGPU_FP32 solve with
+ * mugpu_cg_solve_mixed). */
 int mugpu_cg_start(mugpu_cell cell, const void *d_rhs, void *d_x, void *d_r,
                    void *d_p, void *d_Ap, double *rdr, double *rhs_norm2);
 int mugpu_cg_step(mugpu_cell cell, int update_p, void *d_x, void *d_r,
                   void *d_p, void *d_Ap, double *new_rdr);
+int mugpu_cg_steps(mugpu_cell cell, int update_p, void *d_x, void *d_r,
+                   void *d_p, void *d_Ap, double tol2, int max_steps,
+                   int *steps, double *new_rdr);
 
 /* ---- mixed-precision CG with FP64 refinement (no reference counterpart;
  * it replaces SolverCG::solve for cells created with MUGPU_FP32) --------

@@ -19,6 +19,7 @@ hipStream_t g_stream = nullptr;
 int g_device = -1;
 int g_maxlds = 65536;
 double* g_host_scal = nullptr;  // pinned
+int* g_host_ctl = nullptr;       // pinned, CTL_COUNT words after the scalars
 double* g_dev_scratch = nullptr;  // partials + scalars for mugpu_dot
 }  // namespace
 
@@ -76,6 +77,12 @@ void timing_end(int kclass) {
   ++g_timing_count;
 }
 
+int64_t timing_mark() { return g_timing_count; }
+
+void timing_void(int64_t from) {
+  for (int64_t i = from < 0 ? 0 : from; i < g_timing_count; ++i) g_ev_class[i] = -2;
+}
+
 int copy_device(void* dst, const void* src, size_t bytes);
 
 int run_tangent_product(int dim, int prec, int kind, const void* tan, const void* in, void* out,
@@ -114,6 +121,7 @@ struct mugpu_cell_s {
   std::vector<MaterialDev> mats;
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr;
   double* r64 = nullptr;
+  int* ctl = nullptr;  // CTL_* words of the device-resident inner CG loop
 };
 
 namespace {
@@ -128,6 +136,13 @@ int sync_scalars(const double* d_scal) {
                            g_stream));
   MUGPU_HIP(hipStreamSynchronize(g_stream));
   return MUGPU_OK;
+}
+
+// scalars and the control words of a device-resident CG loop
+int sync_ctl(const int* d_ctl, const double* d_scal) {
+  MUGPU_HIP(hipMemcpyAsync(g_host_ctl, d_ctl, sizeof(int) * CTL_COUNT, hipMemcpyDeviceToHost,
+                           g_stream));
+  return sync_scalars(d_scal);
 }
 
 int check_odd(int dim, const int* res, int allow_even) {
@@ -213,6 +228,35 @@ int cell_operator64(mugpu_cell c, const void* in, void* out) {
   return run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
 }
 
+// control words of the device-resident CG loops (CTL_*)
+int ensure_ctl(mugpu_cell c) {
+  if (!c->ctl && hipMalloc((void**)&c->ctl, sizeof(int) * CTL_COUNT) != hipSuccess) {
+    set_error("device allocation of the CG control words failed");
+    return MUGPU_EMEMORY;
+  }
+  return MUGPU_OK;
+}
+
+// one iteration of the FP32 inner CG (solver_cg.cc:62-96): A p fused with
+// the p update (all but the first iteration), p.Ap, the x and r updates and
+// r.r, gated by the cell's control words
+int inner_iteration(mugpu_cell c, bool first, int64_t n) {
+  OpSpec op;
+  op.in_mode = first ? IN_TAN : IN_TAN_PUPDATE;
+  op.in = c->p32;
+  op.rvec = c->r32;
+  op.pout = c->p32;
+  op.scal = c->scal;
+  op.tan_kind = c->tan_kind;
+  op.tangent = c->tan32;
+  op.out_mode = OUT_DOT;
+  op.out = c->Ap32;
+  op.dotv = c->p32;
+  MUGPU_TRY(run_operator(c->plan, MUGPU_FP32, op, c->pj, c->spectrum));
+  MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzb[1], c->scal, FIN_ALPHA, c->ctl));
+  return cg_update(MUGPU_FP32, c->x32, c->r32, c->p32, c->Ap32, c->scal, n, c->partials, c->ctl);
+}
+
 }  // namespace
 
 // ============================== runtime ======================================
@@ -231,7 +275,9 @@ extern "C" int mugpu_init(int device) {
   MUGPU_HIP(hipSetDevice(device));
   MUGPU_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
   MUGPU_HIP(hipDeviceGetAttribute(&g_maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-  MUGPU_HIP(hipHostMalloc((void**)&g_host_scal, sizeof(double) * SC_COUNT, hipHostMallocDefault));
+  MUGPU_HIP(hipHostMalloc((void**)&g_host_scal, sizeof(double) * SC_COUNT + sizeof(int) * CTL_COUNT,
+                          hipHostMallocDefault));
+  g_host_ctl = reinterpret_cast<int*>(g_host_scal + SC_COUNT);
   MUGPU_HIP(hipMalloc((void**)&g_dev_scratch, sizeof(double) * (kRedBlocks + SC_COUNT)));
   g_device = device;
   return MUGPU_OK;
@@ -311,17 +357,20 @@ extern "C" int mugpu_timing_stop(int64_t* launches, double* total_ms) {
     g_stat_launches[k] = 0;
     g_stat_ms[k] = 0.0;
   }
+  int64_t counted = 0;
   for (int64_t i = 0; i < g_timing_count; ++i) {
+    const int k = g_ev_class[i];
+    if (k == -2) continue;  // a launch that returned at once (device CG loop stopped)
     float ms = 0.f;
     MUGPU_HIP(hipEventElapsedTime(&ms, g_ev_start[i], g_ev_stop[i]));
     tot += ms;
-    const int k = g_ev_class[i];
+    ++counted;
     if (k >= 0 && k < MUGPU_K_ALL) {
       ++g_stat_launches[k];
       g_stat_ms[k] += ms;
     }
   }
-  if (launches) *launches = g_timing_count;
+  if (launches) *launches = counted;
   if (total_ms) *total_ms = tot;
   g_timing_class = -1;
   g_timing_count = 0;
@@ -631,6 +680,7 @@ extern "C" int mugpu_cell_destroy(mugpu_cell c) {
   free_dev(c->p32);
   free_dev(c->Ap32);
   free_dev(c->r64);
+  free_dev(c->ctl);
   plan_free(c->plan);
   delete c;
   return MUGPU_OK;
@@ -1006,6 +1056,50 @@ extern "C" int mugpu_cg_step(mugpu_cell c, int update_p, void* d_x, void* d_r, v
   return MUGPU_OK;
 }
 
+extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, void* d_p,
+                              void* d_Ap, double tol2, int max_steps, int* steps,
+                              double* new_rdr) {
+  if (!c || max_steps < 1 || !steps || !new_rdr) return MUGPU_EARG;
+  const int64_t n = c->N * c->nc;
+  MUGPU_TRY(ensure_ctl(c));
+  MUGPU_TRY(set_ctl(c->ctl, max_steps, c->scal, tol2));
+  OpSpec op;
+  op.in = d_p;
+  op.rvec = d_r;
+  op.pout = d_p;
+  op.scal = c->scal;
+  op.tan_kind = c->tan_kind;
+  op.tangent = c->tan64;
+  op.out_mode = OUT_DOT;
+  op.out = d_Ap;
+  op.dotv = d_p;
+  // every iteration is queued; once the loop test fails on the device the
+  // remaining kernels return at once
+  int rc = MUGPU_OK;
+  std::vector<int64_t> marks;
+  c->plan.active = c->ctl;
+  for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) {
+    marks.push_back(timing_mark());
+    op.in_mode = (i > 0 || update_p) ? IN_TAN_PUPDATE : IN_TAN;
+    rc = run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
+    if (rc == MUGPU_OK)
+      rc = finalize((const double*)c->plan.partials, c->plan.nzb[0], c->scal, FIN_ALPHA, c->ctl);
+    if (rc == MUGPU_OK)
+      rc = cg_update(MUGPU_FP64, d_x, d_r, d_p, d_Ap, c->scal, n, c->partials, c->ctl);
+  }
+  c->plan.active = nullptr;
+  MUGPU_TRY(rc);
+  MUGPU_TRY(sync_ctl(c->ctl, c->scal));
+  const double r = g_host_scal[SC_NEWRDR];
+  // a NaN r.r stops the device loop without counting its iteration; the
+  // host loop counts it
+  *steps = g_host_ctl[CTL_ITER] + (r == r ? 0 : 1);
+  *new_rdr = r;
+  // launches after the loop test failed did no work: left out of the timing
+  if (*steps < (int)marks.size()) timing_void(marks[*steps]);
+  return MUGPU_OK;
+}
+
 // ==================== mixed precision CG with FP64 refinement ===================
 extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, double tol,
                                     int maxiter, double inner_tol, int64_t* iterations,
@@ -1059,29 +1153,42 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     MUGPU_TRY(convert(MUGPU_FP64, c->r64, MUGPU_FP32, c->p32, n, s));
     MUGPU_TRY(dot_device(MUGPU_FP32, c->r32, c->r32, n, c->partials, c->scal, FIN_STORE_RDR));
     MUGPU_TRY(sync_scalars(c->scal));
-    double rdr = g_host_scal[SC_RDR];
-    const double tol2i = inner_tol * inner_tol * rdr;
-    for (int i = 0; its < maxiter && (rdr > tol2i || i == 0); ++i, ++its) {
-      OpSpec op;
-      op.in_mode = i ? IN_TAN_PUPDATE : IN_TAN;
-      op.in = c->p32;
-      op.rvec = c->r32;
-      op.pout = c->p32;
-      op.scal = c->scal;
-      op.tan_kind = c->tan_kind;
-      op.tangent = c->tan32;
-      op.out_mode = OUT_DOT;
-      op.out = c->Ap32;
-      op.dotv = c->p32;
-      MUGPU_TRY(run_operator(c->plan, MUGPU_FP32, op, c->pj, c->spectrum));
-      MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzb[1], c->scal, FIN_ALPHA));
-      MUGPU_TRY(cg_update(MUGPU_FP32, c->x32, c->r32, c->p32, c->Ap32, c->scal, n, c->partials));
-      MUGPU_TRY(sync_scalars(c->scal));
-      rdr = g_host_scal[SC_NEWRDR];
-      if (!(rdr == rdr)) break;
+    const double rdr0 = g_host_scal[SC_RDR];
+    const double tol2i = inner_tol * inner_tol * rdr0;
+    // SolverCG's loop, for (i = 0; its < maxiter && (r.r > tol2i || i == 0);
+    // ++i, ++its), run on the device: iterations are queued kChunk at a time
+    // ahead of the host, the FIN_BETA finalize applies the loop test and
+    // clears ctl[CTL_ACTIVE], after which the remaining kernels of the chunk
+    // return at once; the host reads the counters once per chunk
+    constexpr int kChunk = 8;
+    MUGPU_TRY(ensure_ctl(c));
+    const int maxit = (int)(maxiter - its);
+    MUGPU_TRY(set_ctl(c->ctl, maxit, c->scal, tol2i));
+    int launched = 0;
+    int rc = MUGPU_OK;
+    std::vector<int64_t> marks;
+    c->plan.active = c->ctl;
+    while (rc == MUGPU_OK) {
+      for (int j = 0; j < kChunk && launched < maxit && rc == MUGPU_OK; ++j, ++launched) {
+        marks.push_back(timing_mark());
+        rc = inner_iteration(c, launched == 0, n);
+      }
+      if (rc != MUGPU_OK) break;
+      rc = sync_ctl(c->ctl, c->scal);
+      if (rc != MUGPU_OK || !g_host_ctl[CTL_ACTIVE] || launched >= maxit) break;
     }
+    c->plan.active = nullptr;
+    MUGPU_TRY(rc);
+    // launches after the loop test failed did no work: left out of the timing
+    // (the iteration with a NaN r.r, not counted, did)
+    {
+      const int ran = g_host_ctl[CTL_ITER] + (g_host_scal[SC_NEWRDR] == g_host_scal[SC_NEWRDR] ? 0 : 1);
+      if (ran < (int)marks.size()) timing_void(marks[ran]);
+    }
+    its += g_host_ctl[CTL_ITER];
     MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n));
   }
+
   if (iterations) *iterations = its;
   if (refinements) *refinements = refs;
   if (rel_residual) *rel_residual = bnorm2 > 0 ? std::sqrt(rr / bnorm2) : 0.0;

@@ -36,7 +36,9 @@ __global__ __launch_bounds__(kRT) void k_dot(const A* __restrict__ a, const B* _
 }
 
 __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials,
-                                                   int64_t count, double* scal, int mode) {
+                                                   int64_t count, double* scal, int mode,
+                                                   int* ctl) {
+  if (ctl && !ctl[CTL_ACTIVE]) return;
   __shared__ double red[1024];
   double acc = 0.0;
   for (int64_t i = threadIdx.x; i < count; i += 1024) acc += partials[i];
@@ -51,6 +53,17 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ pa
         scal[SC_NEWRDR] = s;
         scal[SC_BETA] = s / scal[SC_RDR];
         scal[SC_RDR] = s;
+        if (ctl) {
+          // SolverCG's loop test after this iteration; a NaN r.r stops the
+          // loop without counting the iteration, as the host loop's break
+          if (!(s == s)) {
+            ctl[CTL_ACTIVE] = 0;
+          } else {
+            const int it = ctl[CTL_ITER] + 1;
+            ctl[CTL_ITER] = it;
+            if (!(it < ctl[CTL_MAXIT] && s > scal[SC_TOL2])) ctl[CTL_ACTIVE] = 0;
+          }
+        }
         break;
       case FIN_STORE_TMP0: scal[SC_TMP0] = s; break;
       case FIN_STORE_TMP1: scal[SC_TMP1] = s; break;
@@ -65,7 +78,8 @@ __global__ __launch_bounds__(kRT) void k_cg_update(T* __restrict__ x, T* __restr
                                                    const T* __restrict__ p,
                                                    const T* __restrict__ Ap,
                                                    const double* __restrict__ scal, int64_t n,
-                                                   double* partials) {
+                                                   double* partials, const int* ctl) {
+  if (ctl && !ctl[CTL_ACTIVE]) return;
   __shared__ double red[kRT];
   const double alpha = scal[SC_ALPHA];
   const T al = (T)alpha;
@@ -139,6 +153,16 @@ inline unsigned grid_for(int64_t n) {
   return (unsigned)g;
 }
 
+// control words of a device-resident CG loop and its r.r threshold
+__global__ void k_set_ctl(int* ctl, int maxit, double* scal, double tol2) {
+  if (threadIdx.x == 0) {
+    ctl[CTL_ACTIVE] = 1;
+    ctl[CTL_ITER] = 0;
+    ctl[CTL_MAXIT] = maxit;
+    scal[SC_TOL2] = tol2;
+  }
+}
+
 __global__ __launch_bounds__(kRT) void k_copy(const double2* __restrict__ s,
                                               double2* __restrict__ d, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * kRT;
@@ -167,10 +191,14 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const BatchSlot* __rest
       scal[SC_BETA] = s / scal[SC_RDR];
       scal[SC_RDR] = s;
       // loop test of the inner CG: iterations left and r.r above the
-      // threshold (a NaN fails the comparison and stops the solve too)
-      const int it = sl.ctl[CTL_ITER] + 1;
-      sl.ctl[CTL_ITER] = it;
-      if (!(it < sl.ctl[CTL_MAXIT] && s > scal[SC_TOL2])) sl.ctl[CTL_ACTIVE] = 0;
+      // threshold; a NaN r.r stops the solve without counting the iteration
+      if (!(s == s)) {
+        sl.ctl[CTL_ACTIVE] = 0;
+      } else {
+        const int it = sl.ctl[CTL_ITER] + 1;
+        sl.ctl[CTL_ITER] = it;
+        if (!(it < sl.ctl[CTL_MAXIT] && s > scal[SC_TOL2])) sl.ctl[CTL_ACTIVE] = 0;
+      }
     }
   }
 }
@@ -253,25 +281,32 @@ int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_part
   return finalize(d_partials, g, d_scal, mode);
 }
 
-int finalize(const double* d_partials, int64_t count, double* d_scal, int mode) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream(), d_partials, count, d_scal, mode);
+int set_ctl(int* ctl, int maxit, double* d_scal, double tol2) {
+  hipLaunchKernelGGL(k_set_ctl, dim3(1), dim3(64), 0, stream(), ctl, maxit, d_scal, tol2);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int finalize(const double* d_partials, int64_t count, double* d_scal, int mode, int* ctl) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, stream(), d_partials, count, d_scal, mode,
+                     ctl);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }
 
 int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
-              int64_t n, double* d_partials) {
+              int64_t n, double* d_partials, int* ctl) {
   const unsigned g = grid_for(n);
   timing_begin(MUGPU_K_CG_UPDATE);
   if (prec == MUGPU_FP64)
     hipLaunchKernelGGL((k_cg_update<double>), dim3(g), dim3(kRT), 0, stream(), (double*)x,
-                       (double*)r, (const double*)p, (const double*)Ap, d_scal, n, d_partials);
+                       (double*)r, (const double*)p, (const double*)Ap, d_scal, n, d_partials, ctl);
   else
     hipLaunchKernelGGL((k_cg_update<float>), dim3(g), dim3(kRT), 0, stream(), (float*)x,
-                       (float*)r, (const float*)p, (const float*)Ap, d_scal, n, d_partials);
+                       (float*)r, (const float*)p, (const float*)Ap, d_scal, n, d_partials, ctl);
   timing_end(MUGPU_K_CG_UPDATE);
   MUGPU_HIP(hipGetLastError());
-  return finalize(d_partials, g, const_cast<double*>(d_scal), FIN_BETA);
+  return finalize(d_partials, g, const_cast<double*>(d_scal), FIN_BETA, ctl);
 }
 
 int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n) {

@@ -251,7 +251,8 @@ __device__ __forceinline__ void strided_body(c_t<T>* __restrict__ data, int64_t 
 template <typename T, int N, int E, int SG>
 __global__ __launch_bounds__((s_lines<T, N, E>() * (N / E)))
 MUGPU_WPE((s_lines<T, N, E>() * (N / E)), (lds_strided<T, N, E>())) void k_strided_fast(
-    c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw) {
+    c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw, const int* active) {
+  if (active && !active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   strided_body<T, N, E, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
@@ -280,6 +281,7 @@ struct XArgs {
   int hlast;
   double L[3];
   T norm;
+  const int* active;  // FftPlan::active (single-cell kernels only)
 };
 
 template <typename T, int N, int E, int DIM, int H>
@@ -333,6 +335,7 @@ template <typename T, int N, int E, int DIM, int H>
 // (no register bound: the projection stage needs its registers, and spilling
 // them costs more than the second workgroup per CU gains)
 __global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   axis0_body<T, N, E, DIM, H>(a, smem, blockIdx.x, gridDim.x);
 }
@@ -370,6 +373,7 @@ struct LArgs {
   double* partials;
   int64_t rows;
   const c_t<T>* tw;
+  const int* active;  // FftPlan::active (single-cell kernels only)
 };
 
 // ISO: the tangent is known to be isotropic (TAN_ISO), which the compiler
@@ -507,6 +511,7 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
 template <typename T, int N, int E, int DIM, int IN, bool ISO, int H>
 __global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
 MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM, H>())) void k_r2c_fast(LArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   r2c_body<T, N, E, DIM, IN, ISO, H>(a, smem, blockIdx.x);
 }
@@ -638,6 +643,7 @@ __device__ __forceinline__ void c2r_body(const LArgs<T>& a, unsigned char* smem,
 template <typename T, int N, int E, int DIM, int OUT, int H>
 __global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
 MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_c2r<T, N, E, DIM, H>())) void k_c2r_fast(LArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   c2r_body<T, N, E, DIM, OUT, H>(a, smem, blockIdx.x);
 }

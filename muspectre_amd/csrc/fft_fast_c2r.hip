@@ -15,6 +15,7 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.partials = reinterpret_cast<double*>(p.partials);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
+  a.active = p.active;
   constexpr int RB = z_rows(N, DIM, H), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
   constexpr size_t lds = lds_c2r<T, N, E, DIM, H>();

@@ -18,6 +18,7 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.spec = reinterpret_cast<c_t<T>*>(spec);
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
+  a.active = p.active;
   constexpr int RB = z_rows(N, DIM, H);
   constexpr int NT = RB / 2 * DIM * DIM * (N / E);
   constexpr size_t lds = lds_lines<T, N, DIM, H>();

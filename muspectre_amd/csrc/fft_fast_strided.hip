@@ -18,9 +18,11 @@ int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
   c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
   if (sign < 0)
-    hipLaunchKernelGGL((k_strided_fast<T, N, E, -1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+    hipLaunchKernelGGL((k_strided_fast<T, N, E, -1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax),
+                       p.active);
   else
-    hipLaunchKernelGGL((k_strided_fast<T, N, E, +1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax));
+    hipLaunchKernelGGL((k_strided_fast<T, N, E, +1>), grid, dim3(NT), lds, stream(), data, inner, tw_of<T>(p, ax),
+                       p.active);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }

@@ -40,6 +40,7 @@ struct ZArgs {
   int ls;
   AxisPlan ap;
   const c_t<T>* tw;
+  const int* active;  // FftPlan::active
 };
 
 template <int NT>
@@ -60,6 +61,7 @@ __device__ double block_sum(double v, double* red) {
 //   X_a[k] = (Z[k] + conj Z[n-k]) / 2,  X_b[k] = (Z[k] - conj Z[n-k]) / 2i
 template <typename T, int NT, int DIM, int IN>
 __global__ __launch_bounds__(NT) void k_last_r2c(ZArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   T* sr = reinterpret_cast<T*>(smem);
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(NT) void k_last_r2c(ZArgs<T> a) {
 // imaginary part.
 template <typename T, int NT, int DIM, int OUT>
 __global__ __launch_bounds__(NT) void k_last_c2r(ZArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double red[NT];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
@@ -251,10 +254,12 @@ struct SArgs {
   int sign;
   AxisPlan ap;
   const c_t<T>* tw;
+  const int* active;  // FftPlan::active
 };
 
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void k_strided(SArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   const int n = a.ap.n;
@@ -292,10 +297,12 @@ struct PArgs {
   int hlast;
   double L[3];
   T norm;
+  const int* active;  // FftPlan::active
 };
 
 template <typename T, int NT, int DIM>
 __global__ __launch_bounds__(NT) void k_axis0_project(PArgs<T> a) {
+  if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
   c_t<T>* s = reinterpret_cast<c_t<T>*>(smem);
   constexpr int NC = DIM * DIM;
@@ -579,6 +586,7 @@ int launch_r2c(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
   a.spec = spec;
   const int n = p.res[DIM - 1];
   a.rows = p.N / n;
+  a.active = p.active;
   a.rb = p.zrb;
   a.ls = odd_stride(n);
   a.ap = p.ax[DIM - 1];
@@ -613,6 +621,7 @@ int launch_c2r(FftPlan& p, const OpSpec& op, c_t<T>* spec) {
   const int n = p.res[DIM - 1];
   const int nh = n / 2 + 1;
   a.rows = p.N / n;
+  a.active = p.active;
   a.rb = p.zrb;
   a.ls = odd_stride(n);
   a.ap = p.ax[DIM - 1];
@@ -651,6 +660,7 @@ int launch_strided(FftPlan& p, int ax, int W, c_t<T>* spec, int sign) {
     return rc;
   }
   a.inner = inner;
+  a.active = p.active;
   a.W = W;
   a.ls = odd_stride(n);
   a.sign = sign;
@@ -690,6 +700,7 @@ int launch_project(FftPlan& p, const ProjParams& pj, c_t<T>* spec) {
   }
   a.hlast = p.hres[DIM - 1];
   a.norm = (T)pj.norm;
+  a.active = p.active;
   const size_t lds = (size_t)a.Wk * DIM * DIM * a.ls * csize<T>();
   dim3 grid((unsigned)((a.plane + a.Wk - 1) / a.Wk));
   timing_begin(MUGPU_K_AXIS0);

@@ -1,6 +1,7 @@
 // Implementation of the host layer over the C-ABI (include/mugpu.h).
 #include "muspectre_host.hh"
 
+#include <algorithm>
 #include <cmath>
 #include <iomanip>
 #include <iostream>
@@ -461,12 +462,22 @@ DeviceVector& SolverCG::solve(const DeviceVector& rhs) {
   const Real tol2 = tol * tol * rhs_norm2;
   size_t count_width = 0;
   if (verbose) count_width = size_t(std::log10(maxiter)) + 1;
-  for (Uint i = 0; i < maxiter && (rdr > tol2 || i == 0); ++i, ++counter) {
+  // for (i = 0; i < maxiter && (rdr > tol2 || i == 0); ++i, ++counter),
+  // with the iterations queued on the device kChunk at a time and the loop
+  // test applied there after each (one at a time when verbose, to print
+  // every step)
+  constexpr Uint kChunk = 8;
+  for (Uint i = 0; i < maxiter && (rdr > tol2 || i == 0);) {
+    const Uint k = verbose ? 1 : std::min<Uint>(kChunk, maxiter - i);
     double new_rdr = 0;
-    check(mugpu_cg_step(h, i > 0 ? 1 : 0, x_k.data(), r_k.data(), p_k.data(), Ap_k.data(), &new_rdr));
+    int done = 0;
+    check(mugpu_cg_steps(h, i > 0 ? 1 : 0, x_k.data(), r_k.data(), p_k.data(), Ap_k.data(), tol2,
+                         (int)k, &done, &new_rdr));
     rdr = new_rdr;
+    i += (Uint)done;
+    counter += (Uint)done;
     if (verbose)
-      std::cout << "  at CG step " << std::setw(count_width) << i
+      std::cout << "  at CG step " << std::setw(count_width) << i - 1
                 << ": |r|/|b| = " << std::setw(15) << std::sqrt(rdr / rhs_norm2)
                 << ", cg_tol = " << tol << std::endl;
   }

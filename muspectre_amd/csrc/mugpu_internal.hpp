@@ -36,6 +36,10 @@ int max_lds_per_block();
 // per-launch event timing of one kernel class (MUGPU_K_*), off by default
 void timing_begin(int kclass);
 void timing_end(int kclass);
+// position in the timing record, and dropping the launches recorded since a
+// position (launches that returned at once; not counted anywhere)
+int64_t timing_mark();
+void timing_void(int64_t from);
 
 // ---- tangent representations of the directional-stiffness product --------
 // FULL: the reference's stored K, nc*nc entries per voxel, entry r + nc*c is
@@ -101,6 +105,9 @@ struct FftPlan {
   // 1 FP32), per pass and, for the r2c pass, per tangent (0 full matrices,
   // 1 isotropic)
   unsigned char fvar[2][5][2] = {};
+  // while set: every pass kernel returns at once when active[0] == 0 (CG
+  // iterations launched ahead of the host's stopping test)
+  const int* active = nullptr;
 };
 // passes with variants
 enum FastVariantPass { FV_R2C = 0, FV_STRIDED1 = 1, FV_AXIS0 = 2, FV_C2R = 3, FV_STRIDED0 = 4, FV_COUNT = 5 };
@@ -195,11 +202,17 @@ int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots);
 // ---- reductions and vector kernels (cg.hip) --------------------------------
 int dot_device(int prec, const void* a, const void* b, int64_t n, double* d_partials,
                double* d_scal, int slot);
-int finalize(const double* d_partials, int64_t count, double* d_scal, int mode);
+// ctl (optional, CTL_* words): skip when ctl[CTL_ACTIVE] == 0; FIN_BETA then
+// also applies SolverCG's loop test (ctl[CTL_ITER] + 1 < ctl[CTL_MAXIT] and
+// r.r > scal[SC_TOL2]) and clears CTL_ACTIVE when it fails
+int finalize(const double* d_partials, int64_t count, double* d_scal, int mode,
+             int* ctl = nullptr);
+// ctl = {active, 0 iterations, maxit}, scal[SC_TOL2] = tol2 (on the stream)
+int set_ctl(int* ctl, int maxit, double* d_scal, double tol2);
 enum FinalizeMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_STORE_TMP0 = 2, FIN_STORE_TMP1 = 3,
                     FIN_STORE_RDR = 4 };
 int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
-              int64_t n, double* d_partials);
+              int64_t n, double* d_partials, int* ctl = nullptr);
 int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n);
 int axpby(int prec, double a, const void* x, double b, void* y, int64_t n);
 int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, double scale);

@@ -185,3 +185,64 @@ def test_cg_solve_vs_oracle(capi, oracle, case):
     assert conv
     assert it == it_ref
     assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+
+
+def _gpu_cg_chunked(gc, rhs, tol, maxiter, chunk):
+    """The same loop with the iterations queued `chunk` at a time and the
+    loop test applied on the device (mugpu_cg_steps)."""
+    lib, capi = gc.lib, gc.capi
+    n = gc.N * gc.nc
+    b = DeviceBuffer.from_array(rhs)
+    x, r, p, Ap = (DeviceBuffer(8 * n) for _ in range(4))
+    rdr = ctypes.c_double()
+    bn2 = ctypes.c_double()
+    capi.check(lib.mugpu_cg_start(gc.h, b.ptr, x.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(rdr), ctypes.byref(bn2)))
+    tol2 = tol * tol * bn2.value
+    rd = rdr.value
+    it = 0
+    calls = 0
+    while it < maxiter and (rd > tol2 or it == 0):
+        nr = ctypes.c_double()
+        done = ctypes.c_int()
+        k = min(chunk, maxiter - it)
+        capi.check(lib.mugpu_cg_steps(gc.h, 1 if it > 0 else 0, x.ptr, r.ptr, p.ptr, Ap.ptr, tol2, k,
+                                      ctypes.byref(done), ctypes.byref(nr)))
+        assert 1 <= done.value <= k
+        rd = nr.value
+        it += done.value
+        calls += 1
+    return x.download((gc.N, gc.nc)), r.download((gc.N, gc.nc)), it, rd, calls
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[6], CASES[9]], ids=lambda c: c[0])
+@pytest.mark.parametrize("chunk", [1, 3, 8])
+def test_cg_steps_equal_step_loop(capi, case, chunk):
+    """Iterations queued on the device with the loop test there give the
+    host loop's iterates bit for bit, and stop where it stops: at
+    convergence, and at maxiter in the middle of a chunk."""
+    name, res, L, form, f_in, f_out, Ein, Eout = case[:8]
+    even = case[8] if len(case) > 8 else 0
+    d = len(res)
+    gc = GpuCell(capi, res, L, form, allow_even=even)
+    mask = _sphere_mask(res, min(res) / 3)
+    mi = gc.add_material(f_in, Ein, .3)
+    mo = gc.add_material(f_out, Eout, .3)
+    idx = np.arange(mask.size)
+    gc.add_pixels(mi, idx[mask.ravel()])
+    gc.add_pixels(mo, idx[~mask.ravel()])
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    F = np.zeros((gc.N, gc.nc))
+    if form == 0:
+        F += np.eye(d).ravel(order="F")
+    F[:, 1] += 0.01
+    F[:, d] += 0.01
+    gc.set_F(F)
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    rhs = np.random.default_rng(5).standard_normal((gc.N, gc.nc))
+    for maxiter in (400, 7):
+        x1, it1, conv1 = _gpu_cg(gc, rhs, 1e-8, maxiter)
+        x2, _, it2, rd2, calls = _gpu_cg_chunked(gc, rhs, 1e-8, maxiter, chunk)
+        assert it2 == it1
+        assert calls <= -(-it1 // chunk) + 1
+        np.testing.assert_array_equal(x2, x1)
+    assert conv1 is False and it1 == 7
