@@ -91,7 +91,7 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
   int rc = MUGPU_EARG;
   const unsigned char(&fv)[FV_COUNT][2] = p.fvar[1];
   timing_begin(MUGPU_K_LAST_R2C);
-#define C_(n, e) b_r2c_h<n, e, DIM>(p, slots, ns, fv[FV_R2C][1] >> 1)
+#define C_(n, e) b_r2c_h<n, e, DIM>(p, slots, ns, (fv[FV_R2C][1] >> 1) & 1)
   rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_R2C][1] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_R2C);
@@ -105,7 +105,7 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_AXIS0);
-#define C_(n, e) b_axis0_h<n, e, DIM>(p, slots, ns, fv[FV_AXIS0][0] >> 1)
+#define C_(n, e) b_axis0_h<n, e, DIM>(p, slots, ns, (fv[FV_AXIS0][0] >> 1) & 1)
   rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], fv[FV_AXIS0][0] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_AXIS0);
@@ -119,7 +119,7 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_LAST_C2R);
-#define C_(n, e) b_c2r_h<n, e, DIM>(p, slots, ns, fv[FV_C2R][0] >> 1)
+#define C_(n, e) b_c2r_h<n, e, DIM>(p, slots, ns, (fv[FV_C2R][0] >> 1) & 1)
   rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[DIM - 1], fv[FV_C2R][0] & 1, C_) return MUGPU_EARG; }();
 #undef C_
   timing_end(MUGPU_K_LAST_C2R);
