@@ -65,6 +65,10 @@ extern "C" {
  * counterpart: the reference's coded HEP1 tangent, materials_toolbox.hh:
  * 394-406, is not the derivative of P and makes CG diverge) */
 #define MUGPU_HEP1_CT 6
+/* MaterialLinearElasticGeneric (material_linear_elastic_generic.hh): one
+ * stiffness tensor per material, given in Voigt notation; added with
+ * mugpu_cell_add_material_generic */
+#define MUGPU_LEG 7
 
 typedef struct mugpu_fft_s *mugpu_fft;
 typedef struct mugpu_proj_s *mugpu_proj;
@@ -147,6 +151,16 @@ int mugpu_cell_nb_components(mugpu_cell cell);
 int mugpu_cell_add_material(mugpu_cell cell, int family, double young,
                             double poisson, double tau_y0, double H,
                             int *mat);
+/* MaterialLinearElasticGeneric(name, C_voigt)
+ * (src/materials/material_linear_elastic_generic.cc:40-65): C_voigt is the
+ * vsize x vsize stiffness in Voigt notation, row-major (vsize 3 in 2D with
+ * the order xx, yy, xy; 6 in 3D with xx, yy, zz, yz, xz, xy,
+ * src/common/voigt_conversion.hh:118-125); C_ijkl = C_voigt(v(ij), v(kl)).
+ * Green-Lagrange strain to PK2 stress, S = C : E (:164-176); the material
+ * index is returned in *mat and pixels are added with
+ * mugpu_material_add_pixels (no parameters). */
+int mugpu_cell_add_material_generic(mugpu_cell cell, const double *C_voigt,
+                                    int *mat);
 /* MaterialBase::add_pixel and the LE2-LE4 overloads
  * (src/materials/material_base.cc:56-58, material_linear_elastic2.cc:57-64,
  * material_linear_elastic3.cc:57-68, material_linear_elastic4.cc:62-73):

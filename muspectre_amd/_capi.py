@@ -14,7 +14,7 @@ OK, ECONVERGENCE, ESOLVER, EPROJECTION, ERUNTIME, EMEMORY, EHIP, EARG = range(8)
 FP64, FP32 = 0, 1
 FINITE_STRAIN, SMALL_STRAIN = 0, 1
 PROJ_SMALL_STRAIN, PROJ_FINITE_STRAIN_FAST, PROJ_FINITE_STRAIN = 0, 1, 2
-LE1, LE2, LE3, LE4, HEP1, HEP1_CT = 1, 2, 3, 4, 5, 6
+LE1, LE2, LE3, LE4, HEP1, HEP1_CT, LEG = 1, 2, 3, 4, 5, 6, 7
 
 class ServerConfig(ctypes.Structure):
     """mugpu_server_config of include/mugpu.h."""
@@ -61,6 +61,7 @@ SIGNATURES = [
     ("mugpu_cell_nb_pixels", _i64, [_vp]),
     ("mugpu_cell_nb_components", _int, [_vp]),
     ("mugpu_cell_add_material", _int, [_vp, _int, _dbl, _dbl, _dbl, _dbl, _ip]),
+    ("mugpu_cell_add_material_generic", _int, [_vp, _dp, _ip]),
     ("mugpu_material_add_pixels", _int, [_vp, _int, _i64, _i64p, _dp]),
     ("mugpu_cell_initialise", _int, [_vp]),
     ("mugpu_cell_is_initialised", _int, [_vp]),

@@ -661,6 +661,7 @@ extern "C" int mugpu_cell_destroy(mugpu_cell c) {
   for (auto& m : c->mats) {
     free_dev(m.d_pix);
     free_dev(m.d_params);
+    free_dev(m.d_C);
     for (int b = 0; b < 2; ++b) {
       free_dev(m.Fp[b]);
       free_dev(m.be[b]);
@@ -696,6 +697,10 @@ extern "C" int mugpu_cell_add_material(mugpu_cell c, int family, double young, d
     set_error("materials must be added before the cell is initialised");
     return MUGPU_ERUNTIME;
   }
+  if (family == MUGPU_LEG) {
+    set_error("MaterialLinearElasticGeneric takes a stiffness matrix: use mugpu_cell_add_material_generic");
+    return MUGPU_EARG;
+  }
   if (family < MUGPU_LE1 || family > MUGPU_HEP1_CT) {
     set_error("unknown material family %d", family);
     return MUGPU_EARG;
@@ -714,6 +719,33 @@ extern "C" int mugpu_cell_add_material(mugpu_cell c, int family, double young, d
   m.mu = lame_mu(young, poisson);
   m.bulk = bulk_modulus(young, poisson);
   m.ppn = family == MUGPU_LE2 ? c->nc : (family == MUGPU_LE3 || family == MUGPU_LE4) ? 2 : 0;
+  c->mats.push_back(std::move(m));
+  *mat = (int)c->mats.size() - 1;
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_add_material_generic(mugpu_cell c, const double* C_voigt, int* mat) {
+  if (!c || !C_voigt || !mat) return MUGPU_EARG;
+  if (c->initialised) {
+    set_error("materials must be added before the cell is initialised");
+    return MUGPU_ERUNTIME;
+  }
+  // VoigtConversion<dim>::sym_mat (src/common/voigt_conversion.hh:118-125)
+  static const int sym2[2][2] = {{0, 2}, {2, 1}};
+  static const int sym3[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+  const int d = c->dim, nc = c->nc, vs = d == 2 ? 3 : 6;
+  auto sym = [&](int i, int j) { return d == 2 ? sym2[i][j] : sym3[i][j]; };
+  MaterialDev m;
+  m.family = MUGPU_LEG;
+  m.h_C.assign((size_t)nc * nc, 0.0);
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j)
+      for (int k = 0; k < d; ++k)
+        for (int l = 0; l < d; ++l)
+          m.h_C[(i + d * j) + nc * (k + d * l)] = C_voigt[sym(i, j) * vs + sym(k, l)];
+  for (int r = 0; r < nc; ++r)
+    for (int q = 0; q < nc; ++q)
+      if (m.h_C[r + nc * q] != m.h_C[q + nc * r]) m.major_sym = false;
   c->mats.push_back(std::move(m));
   *mat = (int)c->mats.size() - 1;
   return MUGPU_OK;
@@ -772,19 +804,21 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
       return MUGPU_ERUNTIME;
     }
   // tangent representation
-  // small strain: every law here is isotropic Hooke per voxel -> (lambda, mu);
-  // finite strain: the coded HEP1 tangent alone -> its dim^3 form; laws whose
-  // tangent is major-symmetric (LE1-4 pushed forward from PK2, the
+  // small strain: isotropic Hooke per voxel -> (lambda, mu), unless a
+  // generic material brings a general C; finite strain: the coded HEP1
+  // tangent alone -> its dim^3 form; laws whose tangent is major-symmetric
+  // (LE1-4 and a major-symmetric generic C pushed forward from PK2, the
   // consistent HEP1 tangent; 3D) -> upper triangle; otherwise the full matrix
-  bool all_hep = !c->mats.empty(), all_sym = c->dim == 3 && !c->mats.empty();
+  bool all_hep = !c->mats.empty(), all_sym = c->dim == 3 && !c->mats.empty(), any_gen = false;
   for (auto& m : c->mats) {
     all_hep = all_hep && m.family == MUGPU_HEP1;
-    all_sym = all_sym && m.family != MUGPU_HEP1;
+    all_sym = all_sym && m.family != MUGPU_HEP1 && (m.family != MUGPU_LEG || m.major_sym);
+    any_gen = any_gen || m.family == MUGPU_LEG;
   }
-  c->tan_kind = c->form == MUGPU_SMALL_STRAIN ? TAN_ISO
-                : all_hep                     ? TAN_HEP
-                : all_sym                     ? TAN_SYM
-                                              : TAN_FULL;
+  c->tan_kind = c->form == MUGPU_SMALL_STRAIN && !any_gen ? TAN_ISO
+                : all_hep                                 ? TAN_HEP
+                : all_sym                                 ? TAN_SYM
+                                                          : TAN_FULL;
   const size_t tn = (size_t)tangent_values(c->tan_kind, c->dim, c->N);
   if (hipMalloc((void**)&c->tan64, tn * sizeof(double)) != hipSuccess) {
     set_error("device allocation of the tangent failed");
@@ -803,6 +837,10 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
     std::vector<int> pix32(m.h_pix.begin(), m.h_pix.end());
     MUGPU_HIP(hipMalloc((void**)&m.d_pix, sizeof(int) * (size_t)(n > 0 ? n : 1)));
     if (n) MUGPU_HIP(hipMemcpy(m.d_pix, pix32.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (m.family == MUGPU_LEG) {
+      MUGPU_HIP(hipMalloc((void**)&m.d_C, sizeof(double) * m.h_C.size()));
+      MUGPU_HIP(hipMemcpy(m.d_C, m.h_C.data(), sizeof(double) * m.h_C.size(), hipMemcpyHostToDevice));
+    }
     if (m.ppn) {
       MUGPU_HIP(hipMalloc((void**)&m.d_params, sizeof(double) * (size_t)(n > 0 ? n * m.ppn : 1)));
       if (n) MUGPU_HIP(hipMemcpy(m.d_params, m.h_params.data(), sizeof(double) * n * m.ppn,

@@ -90,6 +90,7 @@ template <Dim_t Dim, int Fam>
 struct MaterialT : MaterialBase {
   MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H, int fam = Fam)
       : MaterialBase(c, n, fam, E, nu, tau, H) {}
+  MaterialT(Cell& c, const std::string& n, const Matrix& C_voigt) : MaterialBase(c, n, C_voigt) {}
 };
 
 template <Dim_t Dim, int Fam>
@@ -111,6 +112,23 @@ void bind_material(py::module& m, const std::string& stem) {
           return attach<Dim, Fam>(sys, new Mat(sys, n, E, nu, 0, 0));
         },
         "cell"_a, "name"_a, "Young"_a, "Poisson"_a, py::return_value_policy::reference,
+        py::keep_alive<1, 0>());
+  } else if (Fam == MUGPU_LEG) {
+    // MaterialLinearElasticGeneric(name, C_voigt)
+    // (material_linear_elastic_generic.cc:40-65; not bound in the reference)
+    cls.def_static(
+        "make",
+        [](Sys& sys, std::string n, rarr Cv) -> Mat& {
+          if (Cv.ndim() != 2) throw std::runtime_error("C_voigt must be a matrix");
+          Matrix C;
+          C.rows = (Dim_t)Cv.shape(0);
+          C.cols = (Dim_t)Cv.shape(1);
+          C.data.resize((size_t)C.rows * C.cols);
+          for (Dim_t j = 0; j < C.cols; ++j)
+            for (Dim_t i = 0; i < C.rows; ++i) C.data[i + C.rows * j] = Cv.at(i, j);
+          return attach<Dim, Fam>(sys, new Mat(sys, n, C));
+        },
+        "cell"_a, "name"_a, "C_voigt"_a, py::return_value_policy::reference,
         py::keep_alive<1, 0>());
   } else if (Fam == MUGPU_HEP1) {
     // material_hyper_elasto_plastic1.cc:40-65 (not bound in the reference);
@@ -474,6 +492,8 @@ PYBIND11_MODULE(_muSpectre, mod) {
   bind_material<3, MUGPU_LE3>(material, "MaterialLinearElastic3");
   bind_material<2, MUGPU_LE4>(material, "MaterialLinearElastic4");
   bind_material<3, MUGPU_LE4>(material, "MaterialLinearElastic4");
+  bind_material<2, MUGPU_LEG>(material, "MaterialLinearElasticGeneric");
+  bind_material<3, MUGPU_LEG>(material, "MaterialLinearElasticGeneric");
   bind_material<2, MUGPU_HEP1>(material, "MaterialHyperElastoPlastic1");
   bind_material<3, MUGPU_HEP1>(material, "MaterialHyperElastoPlastic1");
 

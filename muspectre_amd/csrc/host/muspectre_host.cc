@@ -399,6 +399,24 @@ MaterialBase::MaterialBase(Cell& c, std::string n, int fam, Real E, Real nu, Rea
   nb_params = fam == MUGPU_LE2 ? nc : (fam == MUGPU_LE3 || fam == MUGPU_LE4) ? 2 : 0;
 }
 
+MaterialBase::MaterialBase(Cell& c, std::string n, const Matrix& C_voigt)
+    : cell(c), name(std::move(n)), family(MUGPU_LEG), nb_params(0) {
+  const Dim_t vs = c.get_material_dim() == 2 ? 3 : 6;
+  if (C_voigt.rows != vs || C_voigt.cols != vs) {
+    // the reference builds this message (material_linear_elastic_generic.cc:
+    // 51-58) but goes on with the wrong-sized matrix; here it is raised
+    std::stringstream err;
+    err << "The stiffness tensor should be input as a " << vs << " × " << vs
+        << " Matrix in Voigt notation. You supplied a " << C_voigt.rows << " × " << C_voigt.cols
+        << " matrix";
+    throw std::runtime_error(err.str());
+  }
+  std::vector<Real> rm((size_t)vs * vs);
+  for (Dim_t i = 0; i < vs; ++i)
+    for (Dim_t j = 0; j < vs; ++j) rm[(size_t)i * vs + j] = C_voigt(i, j);
+  check(mugpu_cell_add_material_generic(c.handle(), rm.data(), &mat));
+}
+
 void MaterialBase::add_pixel_index(int64_t index) {
   if (nb_params) {
     throw std::runtime_error(family == MUGPU_LE2

@@ -311,6 +311,9 @@ class MaterialBase {
  public:
   MaterialBase(Cell& cell, std::string name, int family, Real young, Real poisson, Real tau_y0,
                Real H);
+  // MaterialLinearElasticGeneric (material_linear_elastic_generic.cc:40-65):
+  // one stiffness in Voigt notation, vsize x vsize
+  MaterialBase(Cell& cell, std::string name, const Matrix& C_voigt);
   virtual ~MaterialBase() = default;
   const std::string& get_name() const { return name; }
   int get_family() const { return family; }

@@ -388,6 +388,52 @@ __device__ void store_iso(const MatArgs& a, int64_t v, double lam, double mu) {
   }
 }
 
+// S = C : E for a general C, C_ijkl at (i + D j) + D^2 (k + D l)
+// (Matrices::tensmult, as material_linear_elastic_generic.hh:164-176)
+template <int D>
+__device__ __forceinline__ void general_contract(const double* __restrict__ C, const double* E,
+                                                 double* S) {
+  constexpr int NC = D * D;
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    double s = 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) s += C[r + NC * c] * E[c];
+    S[r] = s;
+  }
+}
+
+// small strain: the tangent is C itself
+template <int D>
+__device__ void store_general(const MatArgs& a, int64_t v, const double* __restrict__ C) {
+  constexpr int NC = D * D;
+  for (int c = 0; c < NC; ++c)
+    for (int r = 0; r < NC; ++r) put_K<D>(a, v, r, c, C[r + NC * c]);
+}
+
+// PK2 -> PK1 with a general C (materials_toolbox.hh:332-349)
+template <int D>
+__device__ void pk1_from_pk2_store_general(const MatArgs& a, int64_t v, const double* F,
+                                           const double* S, const double* __restrict__ C) {
+  constexpr int NC = D * D;
+  double* P = a.P + v * NC;
+  double Pl[NC];
+  matmul<D>(F, S, Pl);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) P[q] = Pl[q];
+  if (!a.need_tangent) return;
+  for (int n = 0; n < D; ++n)
+    for (int j = 0; j < D; ++j)
+      for (int m = 0; m < D; ++m)
+        for (int i = 0; i < D; ++i) {
+          double k = (i == j) ? S[m + D * n] : 0.0;
+          for (int r = 0; r < D; ++r)
+            for (int s = 0; s < D; ++s)
+              k += F[i + D * r] * C[(r + D * m) + NC * (n + D * s)] * F[j + D * s];
+          put_K<D>(a, v, i + D * m, j + D * n, k);
+        }
+}
+
 // PK2 -> PK1 with C = lambda Itrac + 2 mu Isymm (materials_toolbox.hh:332-349)
 template <int D>
 __device__ void pk1_from_pk2_store(const MatArgs& a, int64_t v, const double* F, const double* S,
@@ -433,6 +479,18 @@ __global__ __launch_bounds__(256) void k_linear_elastic(MatArgs a) {
   if (FAM == MUGPU_LE2) {
 #pragma unroll
     for (int c = 0; c < NC; ++c) E[c] -= a.params[q * NC + c];
+  }
+  if (FAM == MUGPU_LEG) {
+    // one C for the whole material (a.params)
+    general_contract<D>(a.params, E, S);
+    if (FORM == MUGPU_SMALL_STRAIN) {
+#pragma unroll
+      for (int c = 0; c < NC; ++c) a.P[v * NC + c] = S[c];
+      if (a.need_tangent) store_general<D>(a, v, a.params);
+    } else {
+      pk1_from_pk2_store_general<D>(a, v, F, S, a.params);
+    }
+    return;
   }
   if (FAM == MUGPU_LE1 || FAM == MUGPU_LE2) hooke_stress<D>(lam, mu, E, S);
   else iso_contract<D>(lam, mu, E, S);
@@ -591,6 +649,7 @@ int launch_le(const MaterialDev& m, MatArgs& a, dim3 grid) {
     case MUGPU_LE1: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE1, FORM>), grid, dim3(256), 0, stream(), a); break;
     case MUGPU_LE2: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE2, FORM>), grid, dim3(256), 0, stream(), a); break;
     case MUGPU_LE3: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE3, FORM>), grid, dim3(256), 0, stream(), a); break;
+    case MUGPU_LEG: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LEG, FORM>), grid, dim3(256), 0, stream(), a); break;
     default: hipLaunchKernelGGL((k_linear_elastic<D, MUGPU_LE4, FORM>), grid, dim3(256), 0, stream(), a); break;
   }
   MUGPU_HIP(hipGetLastError());
@@ -613,7 +672,7 @@ int eval_material(const MaterialDev& m, int dim, int form, const double* F, doub
   a.bulk = m.bulk;
   a.tau_y0 = m.tau_y0;
   a.H = m.H;
-  a.params = m.d_params;
+  a.params = m.family == MUGPU_LEG ? m.d_C : m.d_params;
   a.F = F;
   a.P = P;
   a.tan = tangent;

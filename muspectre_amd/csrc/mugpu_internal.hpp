@@ -229,6 +229,9 @@ struct MaterialDev {
   std::vector<int64_t> h_pix;
   std::vector<double> h_params;  // per pixel
   int ppn = 0;
+  std::vector<double> h_C;       // MUGPU_LEG: C_ijkl at (i + d j) + d^2 (k + d l)
+  double* d_C = nullptr;
+  bool major_sym = true;         // MUGPU_LEG: C_ijkl == C_klij
   int* d_pix = nullptr;
   double* d_params = nullptr;
   double* Fp[2] = {nullptr, nullptr};

@@ -311,6 +311,25 @@ int mo_cell_add_material(mo_cell *c, int family, double young, double poisson,
   return c->nmat++;
 }
 
+/* material_linear_elastic_generic.cc:46-65: C(i,j,k,l) = C_voigt(v(i,j), v(k,l))
+ * with v = VoigtConversion<dim>::sym_mat (voigt_conversion.hh:118-125) */
+int mo_cell_add_material_generic(mo_cell *c, const double *C_voigt) {
+  static const int v2[2][2] = {{0, 2}, {2, 1}};
+  static const int v3[3][3] = {{0, 5, 4}, {5, 1, 3}, {4, 3, 2}};
+  const int d = c->dim, nc = c->nc, vs = d == 2 ? 3 : 6;
+  int mi = mo_cell_add_material(c, MO_LEG, 0.0, 0.0, 0.0, 0.0);
+  if (mi < 0) return mi;
+  mo_material *m = &c->mats[mi];
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j)
+      for (int k = 0; k < d; ++k)
+        for (int l = 0; l < d; ++l) {
+          const int a = d == 2 ? v2[i][j] : v3[i][j], b = d == 2 ? v2[k][l] : v3[k][l];
+          m->C[(i + d * j) + nc * (k + d * l)] = C_voigt[a * vs + b];
+        }
+  return mi;
+}
+
 /* material_base.cc:56-58 and the LE2-4 add_pixel overloads */
 int mo_material_add_pixel(mo_cell *c, int mi, long pixel, const double *params) {
   if (mi < 0 || mi >= c->nmat || pixel < 0 || pixel >= c->N) {
@@ -433,6 +452,13 @@ static int eval_materials(mo_cell *c, int need_tangent) {
           if (m->family == MO_LE2)
             for (int k = 0; k < nc; ++k) strain[k] -= m->pp[q * nc + k];
           mo_hooke_stress(d, lam, mu, strain, S);
+          break;
+        }
+        case MO_LEG: {
+          /* material_linear_elastic_generic.hh:164-176: S = C : E */
+          if (c->form == MO_SMALL_STRAIN) memcpy(strain, F, sizeof(double) * nc);
+          else mo_green_lagrange(d, F, strain);
+          mo_tensmult(d, m->C, strain, S);
           break;
         }
         case MO_LE3:

@@ -50,8 +50,10 @@ enum {
 
 /* MO_HEP1_CT: the HEP1 stress update with the consistent tangent dP/dF
  * (no reference counterpart; see mo_hep1_consistent_tangent) */
+/* MO_LEG: MaterialLinearElasticGeneric (material_linear_elastic_generic.hh),
+ * one stiffness per material from a Voigt matrix */
 enum {
-  MO_LE1 = 1, MO_LE2 = 2, MO_LE3 = 3, MO_LE4 = 4, MO_HEP1 = 5, MO_HEP1_CT = 6
+  MO_LE1 = 1, MO_LE2 = 2, MO_LE3 = 3, MO_LE4 = 4, MO_HEP1 = 5, MO_HEP1_CT = 6, MO_LEG = 7
 };
 
 /* threads used by the OpenMP loops (1 = the reference's single-core path) */
@@ -132,6 +134,11 @@ int mo_cell_nb_components(const mo_cell *c);
 /* returns material index >= 0, or <0 on error */
 int mo_cell_add_material(mo_cell *c, int family, double young,
                          double poisson, double tau_y0, double H);
+/* MaterialLinearElasticGeneric(name, C_voigt), material_linear_elastic_generic.cc:
+ * 40-65: C_voigt is vsize x vsize row-major (3 in 2D: xx, yy, xy; 6 in 3D:
+ * xx, yy, zz, yz, xz, xy, voigt_conversion.hh:118-125); returns the material
+ * index >= 0, or <0 on error */
+int mo_cell_add_material_generic(mo_cell *c, const double *C_voigt);
 /* params: LE2 eigenstrain (dim*dim col-major); LE3/LE4 (Young, Poisson) */
 int mo_material_add_pixel(mo_cell *c, int mat, long pixel,
                           const double *params);

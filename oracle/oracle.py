@@ -20,7 +20,7 @@ _LIB = None
 
 FINITE_STRAIN, SMALL_STRAIN = 0, 1
 PROJ_SMALL, PROJ_FINITE_FAST, PROJ_FINITE = 0, 1, 2
-LE1, LE2, LE3, LE4, HEP1, HEP1_CT = 1, 2, 3, 4, 5, 6
+LE1, LE2, LE3, LE4, HEP1, HEP1_CT, LEG = 1, 2, 3, 4, 5, 6, 7
 
 ERRORS = {1: "ConvergenceError", 2: "SolverError", 3: "ProjectionError",
           4: "RuntimeError", 5: "MemoryError"}
@@ -51,6 +51,7 @@ def lib():
                                            ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double]
         L.mo_material_add_pixel.argtypes = [vp, ctypes.c_int, ctypes.c_long, dp]
+        L.mo_cell_add_material_generic.argtypes = [vp, dp]
         for name in ("mo_cell_initialise", "mo_cell_evaluate_stress_tangent",
                      "mo_cell_evaluate_stress", "mo_cell_save_history"):
             getattr(L, name).argtypes = [vp]
@@ -297,6 +298,15 @@ class Cell:
     def add_material(self, family, young=0.0, poisson=0.0, tau_y0=0.0, H=0.0):
         m = lib().mo_cell_add_material(self._h, family, young, poisson,
                                        tau_y0, H)
+        if m < 0:
+            _check(-m)
+        return m
+
+    def add_material_generic(self, C_voigt):
+        """MaterialLinearElasticGeneric: C_voigt is the vsize x vsize Voigt
+        stiffness (material_linear_elastic_generic.cc:40-65)."""
+        cv = np.ascontiguousarray(C_voigt, dtype=np.float64)
+        m = lib().mo_cell_add_material_generic(self._h, _dp(cv))
         if m < 0:
             _check(-m)
         return m

@@ -246,3 +246,88 @@ def test_cg_steps_equal_step_loop(capi, case, chunk):
         assert calls <= -(-it1 // chunk) + 1
         np.testing.assert_array_equal(x2, x1)
     assert conv1 is False and it1 == 7
+
+
+def _aniso_voigt(dim, seed, major_sym=True):
+    vs = 3 if dim == 2 else 6
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((vs, vs))
+    C = 50e9 * (A @ A.T / vs + np.eye(vs))
+    if not major_sym:
+        C += 5e9 * rng.standard_normal((vs, vs))
+    return C
+
+
+# (name, resolutions, formulation, major-symmetric C, allow_even)
+GENERIC = [
+    ("3d_small", [9, 7, 11], 1, True, 0),
+    ("3d_finite", [5, 7, 5], 0, True, 0),
+    ("3d_finite_unsym", [5, 7, 5], 0, False, 0),
+    ("2d_small", [15, 13], 1, True, 0),
+    ("2d_finite_unsym", [7, 9], 0, False, 0),
+    ("3d_small_pow2", [32, 16, 64], 1, True, 1),
+    ("3d_finite_pow2", [16, 32, 16], 0, True, 1),
+]
+
+
+@pytest.mark.parametrize("case", GENERIC, ids=lambda c: c[0])
+def test_generic_material_vs_oracle(capi, oracle, case):
+    """MaterialLinearElasticGeneric (material_linear_elastic_generic.hh)
+    next to LE1: stress, tangent, directional stiffness and a CG solve
+    against the oracle; small strain then needs the full (not isotropic)
+    tangent representation, and a C without major symmetry the full matrix."""
+    name, res, form, msym, even = case
+    d = len(res)
+    nc = d * d
+    L = [1.0] * d
+    Cv = _aniso_voigt(d, 7, msym)
+    oc = oracle.Cell(res, L, form, allow_even=bool(even))
+    gc = GpuCell(capi, res, L, form, allow_even=even)
+    mask = _sphere_mask(res, min(res) / 3)
+    idx = np.arange(mask.size)
+    mg_o = oc.add_material_generic(Cv)
+    ml_o = oc.add_material(1, 70e9, .3)
+    cva = np.ascontiguousarray(Cv)
+    mg = ctypes.c_int()
+    capi.check(gc.lib.mugpu_cell_add_material_generic(gc.h, _arr(cva), ctypes.byref(mg)))
+    ml = gc.add_material(1, 70e9, .3)
+    for i in idx:
+        oc.add_pixel(mg_o if mask.flat[i] else ml_o, int(i))
+    gc.add_pixels(mg.value, idx[mask.ravel()])
+    gc.add_pixels(ml, idx[~mask.ravel()])
+    oc.initialise()
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    rng = np.random.default_rng(3)
+    F = 0.01 * rng.standard_normal((gc.N, nc))
+    if form == 0:
+        F += np.eye(d).ravel(order="F")
+    oc.F[:] = F
+    gc.set_F(F)
+    oc.evaluate_stress_tangent()
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    P = gc.get("P")
+    assert np.abs(P - oc.P).max() <= 1e-12 * np.abs(oc.P).max()
+    K = np.zeros((gc.N, nc * nc))
+    capi.check(gc.lib.mugpu_cell_get_tangent(gc.h, _arr(K)))
+    assert np.abs(K - oc.K).max() <= 1e-12 * np.abs(oc.K).max()
+    dF = rng.standard_normal((gc.N, nc))
+    ref = oc.directional_stiffness(dF)
+    bi = DeviceBuffer.from_array(dF)
+    bo = DeviceBuffer(dF.nbytes)
+    capi.check(gc.lib.mugpu_cell_directional_stiffness(gc.h, bi.ptr, bo.ptr))
+    assert np.abs(bo.download(dF.shape) - ref).max() <= 1e-12 * np.abs(ref).max()
+    if msym:
+        rhs = -oc.apply_projection(oc.P.copy())
+        start = oc.counter.value
+        x_ref = oc.cg_solve(rhs, 1e-8, 400)
+        it_ref = oc.counter.value - start
+        x, it, conv = _gpu_cg(gc, rhs, 1e-8, 400)
+        assert conv and it == it_ref
+        assert np.linalg.norm(x - x_ref) <= 1e-10 * np.linalg.norm(x_ref)
+
+
+def test_generic_material_errors(capi):
+    gc = GpuCell(capi, [5, 5], [1.0, 1.0], 1)
+    m = ctypes.c_int()
+    with pytest.raises(capi.MugpuError):
+        capi.check(gc.lib.mugpu_cell_add_material(gc.h, 7, 1.0, .3, 0.0, 0.0, ctypes.byref(m)))

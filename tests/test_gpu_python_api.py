@@ -270,3 +270,41 @@ def test_projection_api_vs_oracle(µ, oracle):
         out = proj.apply_projection(f)
         ref = oracle.apply_projection(kind, res, L, np.asfortranarray(f).T.reshape(-1, 9)).T
         assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("form", ["small_strain", "finite_strain"])
+def test_generic_material_equals_le1(µ, form):
+    """MaterialLinearElasticGeneric_3d.make(cell, name, C_voigt) with the
+    Voigt matrix of an isotropic law solves exactly as LE1 with the same
+    moduli (material_linear_elastic_generic.hh:164-176 vs
+    material_linear_elastic1.hh); a wrong-sized matrix raises."""
+    res = [7, 9, 5]
+    lam, mu = 2e9, 1.5e9
+    E, nu = mu * (3 * lam + 2 * mu) / (lam + mu), lam / (2 * (lam + mu))
+    Cv = np.zeros((6, 6))
+    Cv[:3, :3] = lam
+    Cv[:3, :3] += 2 * mu * np.eye(3)
+    Cv[3:, 3:] = mu * np.eye(3)
+    F = getattr(µ.Formulation, form)
+    out = []
+    for generic in (True, False):
+        cell = µ.Cell(res, [1.] * 3, F)
+        if generic:
+            soft = µ.material.MaterialLinearElasticGeneric_3d.make(cell, "soft", Cv)
+        else:
+            soft = µ.material.MaterialLinearElastic1_3d.make(cell, "soft", E, nu)
+        hard = µ.material.MaterialLinearElastic1_3d.make(cell, "hard", 5 * E, nu)
+        for i, pix in enumerate(cell):
+            (hard if i % 5 == 0 else soft).add_pixel(pix)
+        cell.initialise()
+        solver = µ.solvers.SolverCG(cell, 1e-8, 200, False)
+        g = np.zeros((3, 3))
+        g[0, 1] = g[1, 0] = 0.01
+        r = µ.solvers.newton_cg(cell, g, solver, 1e-6, 1e-10, 0)
+        out.append((np.asarray(r.grad).copy(), np.asarray(r.stress).copy(), r.nb_fev))
+    assert out[0][2] == out[1][2]
+    assert np.abs(out[0][0] - out[1][0]).max() <= 1e-12 * np.abs(out[1][0]).max()
+    assert np.abs(out[0][1] - out[1][1]).max() <= 1e-10 * np.abs(out[1][1]).max()
+    cell = µ.Cell(res, [1.] * 3, F)
+    with pytest.raises(RuntimeError):
+        µ.material.MaterialLinearElasticGeneric_3d.make(cell, "bad", np.eye(3))

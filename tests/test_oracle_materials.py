@@ -200,3 +200,110 @@ def test_hep1_coded_tangent_is_j_independent(oracle):
     _, K = oracle.hep1_stress_and_tangent(3, E, nu, ty, H, F, np.eye(3), np.eye(3), 0.0, False)
     K4 = K.reshape(3, 3, 3, 3, order="F")  # (i, m, j, n)
     assert np.abs(K4 - K4[:, :, :1, :]).max() == 0.0
+
+
+def _generic_cell(oracle, dim, form, Cv, F):
+    """A 3^dim cell of one MaterialLinearElasticGeneric under a uniform F;
+    returns P and K of pixel 0."""
+    c = oracle.Cell([3] * dim, [1.0] * dim, form)
+    m = c.add_material_generic(Cv)
+    for p in range(3 ** dim):
+        c.add_pixel(m, p)
+    c.initialise()
+    c.F[:] = np.asarray(F).ravel(order="F")
+    c.evaluate_stress_tangent()
+    nc = dim * dim
+    # copies: the views die with the cell
+    return (np.array(c.P[0]).reshape(dim, dim, order="F"),
+            np.array(c.K[0]).reshape(nc, nc, order="F"))
+
+
+def _voigt_iso(dim, lam, mu):
+    """test_material_linear_elastic_generic.cc:63-72: lambda + 2 mu on the
+    normal block diagonal, lambda off it, mu on the shear diagonal."""
+    vs = 3 if dim == 2 else 6
+    C = np.zeros((vs, vs))
+    C[:dim, :dim] = lam
+    C[:dim, :dim] += 2 * mu * np.eye(dim)
+    C[dim:, dim:] += mu * np.eye(vs - dim)
+    return C
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_generic_isotropic_voigt_is_hooke(oracle, dim):
+    """Golden case of test_material_linear_elastic_generic.cc:52-92: the Voigt
+    matrix of lambda = 2, mu = 1.5 gives Hooke's C_ijkl = lambda d_ij d_kl +
+    mu (d_ik d_jl + d_il d_jk); and the generic law equals LE1 with the same
+    moduli in both formulations."""
+    lam, mu = 2.0, 1.5
+    I = np.eye(dim)
+    nc = dim * dim
+    Ch = np.zeros((nc, nc))
+    for i in range(dim):
+        for j in range(dim):
+            for k in range(dim):
+                for l in range(dim):
+                    Ch[i + dim * j, k + dim * l] = lam * I[i, j] * I[k, l] + mu * (I[i, k] * I[j, l] + I[i, l] * I[j, k])
+    _, K = _generic_cell(oracle, dim, 1, _voigt_iso(dim, lam, mu), np.zeros((dim, dim)))
+    assert np.abs(K - Ch).max() < 1e-14
+    E, nu = mu * (3 * lam + 2 * mu) / (lam + mu), lam / (2 * (lam + mu))
+    F = I + 0.04 * np.random.default_rng(9).standard_normal((dim, dim))
+    # small strain: a symmetric strain (C : eps drops the skew part, which
+    # LE1's lambda tr(eps) I + 2 mu eps keeps)
+    eps = 0.5 * ((F - I) + (F - I).T)
+    for form in (0, 1):
+        G = F if form == 0 else eps
+        P, K = _generic_cell(oracle, dim, form, _voigt_iso(dim, lam, mu), G)
+        Pl, Kl = oracle.le1_eval(dim, form, E, nu, G)
+        assert np.abs(P - Pl).max() < 1e-13 * np.abs(Pl).max()
+        assert np.abs(K - Kl).max() < 1e-13 * np.abs(Kl).max()
+
+
+def _voigt_strain(e):
+    """[e11, e22, (e33,) 2e23, 2e13, 2e12] (engineering shears)"""
+    if e.shape[0] == 2:
+        return np.array([e[0, 0], e[1, 1], 2 * e[0, 1]])
+    return np.array([e[0, 0], e[1, 1], e[2, 2], 2 * e[1, 2], 2 * e[0, 2], 2 * e[0, 1]])
+
+
+def _voigt_stress(s):
+    if s.shape[0] == 2:
+        return np.array([s[0, 0], s[1, 1], s[0, 1]])
+    return np.array([s[0, 0], s[1, 1], s[2, 2], s[1, 2], s[0, 2], s[0, 1]])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_generic_anisotropic_voigt_relation(oracle, dim):
+    """Small strain: sigma_voigt = C_voigt eps_voigt with engineering shear
+    strains, the textbook meaning of a Voigt stiffness (fixes the component
+    order xx, yy, zz, yz, xz, xy and the shear factors); the tangent
+    contracted with eps gives sigma."""
+    vs = 3 if dim == 2 else 6
+    A = np.random.default_rng(dim).standard_normal((vs, vs))
+    Cv = A @ A.T + vs * np.eye(vs)
+    e = 0.01 * np.random.default_rng(10 + dim).standard_normal((dim, dim))
+    e = 0.5 * (e + e.T)
+    P, K = _generic_cell(oracle, dim, 1, Cv, e)
+    assert np.abs(P - P.T).max() < 1e-15
+    assert np.abs(_voigt_stress(P) - Cv @ _voigt_strain(e)).max() < 1e-14 * np.abs(Cv).max()
+    assert np.abs(K @ e.ravel(order="F") - P.ravel(order="F")).max() < 1e-14 * np.abs(Cv).max()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_generic_finite_strain(oracle, dim):
+    """Finite strain: S from the Voigt relation on the Green-Lagrange strain,
+    P = F S, and K = dP/dF by finite differences."""
+    vs = 3 if dim == 2 else 6
+    A = np.random.default_rng(20 + dim).standard_normal((vs, vs))
+    Cv = A @ A.T + vs * np.eye(vs)
+    F = np.eye(dim) + 0.03 * np.random.default_rng(30 + dim).standard_normal((dim, dim))
+    P, K = _generic_cell(oracle, dim, 0, Cv, F)
+    E = 0.5 * (F.T @ F - np.eye(dim))
+    Sv = Cv @ _voigt_strain(E)
+    S = np.zeros((dim, dim))
+    idx = [(0, 0), (1, 1), (0, 1)] if dim == 2 else [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    for a, (i, j) in enumerate(idx):
+        S[i, j] = S[j, i] = Sv[a]
+    assert np.abs(P - F @ S).max() < 1e-13 * np.abs(P).max()
+    Kfd = _fd_tangent(lambda G: _generic_cell(oracle, dim, 0, Cv, G)[0], F)
+    assert np.abs(K - Kfd).max() < 1e-6 * np.abs(K).max()
