@@ -86,6 +86,14 @@ int64_t pixel_index(Cell& cell, const Ccoord_t<Dim>& pix) {
 }
 
 // ---- per-family material classes (types for the python names) ----------------
+template <class Solver>
+void add_eigen_solver(py::module& m, const char* name) {
+  py::class_<Solver, SolverEigenBase>(m, name)
+      .def(py::init<Cell&, Real, Uint, bool>(), "cell"_a, "tol"_a, "maxiter"_a, "verbose"_a = false,
+           py::keep_alive<1, 2>())
+      .def("name", &Solver::get_name);
+}
+
 template <Dim_t Dim, int Fam>
 struct MaterialT : MaterialBase {
   MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H, int fam = Fam)
@@ -525,6 +533,13 @@ PYBIND11_MODULE(_muSpectre, mod) {
       .def("name", &SolverCG::get_name)
       .def("set_inner_tol", &SolverCG::set_inner_tol, "tol"_a)
       .def("get_refinements", &SolverCG::get_refinements);
+  // Eigen's iterative solvers (bind_py_solvers.cc:52-72)
+  py::class_<SolverEigenBase, SolverBase>(solvers, "SolverEigenBase");
+  add_eigen_solver<SolverCGEigen>(solvers, "SolverCGEigen");
+  add_eigen_solver<SolverGMRESEigen>(solvers, "SolverGMRESEigen");
+  add_eigen_solver<SolverBiCGSTABEigen>(solvers, "SolverBiCGSTABEigen");
+  add_eigen_solver<SolverDGMRESEigen>(solvers, "SolverDGMRESEigen");
+  add_eigen_solver<SolverMINRESEigen>(solvers, "SolverMINRESEigen");
   // single-step overload first: kwarg equil_tol (bind_py_solvers.cc:82-94),
   // list overload: kwarg equilibrium_tol (:95-104)
   solvers.def(

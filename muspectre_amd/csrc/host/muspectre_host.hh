@@ -381,6 +381,76 @@ class SolverCG : public SolverBase {
   int64_t refinements{0};
 };
 
+// ---- Eigen's iterative solvers (src/solver/solver_eigen.hh:58-225) ----------
+// The reference binds Eigen 3.3's ConjugateGradient, BiCGSTAB, GMRES, DGMRES
+// and MINRES, each with the identity preconditioner, to the cell through a
+// sparse-matrix adaptor whose product is the projected directional
+// stiffness (cell_base.hh:534-591).  Here the same algorithms run on the
+// device operator and device vectors; the host keeps the scalar recurrences
+// (and, for GMRES, the small Hessenberg system).  solve() follows
+// SolverEigen::solve (solver_eigen.cc:61-86): zero initial guess, counter +=
+// iterations, ConvergenceError when the relative residual stays above tol.
+class SolverEigenBase : public SolverBase {
+ public:
+  SolverEigenBase(Cell& cell, Real tol, Uint maxiter, bool verbose = false);
+  DeviceVector& solve(const DeviceVector& rhs) override;
+
+ protected:
+  // one solve from x = 0; iters and the relative residual error out;
+  // false on a numerical breakdown
+  virtual bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) = 0;
+  void apply(const DeviceVector& in, DeviceVector& out);  // A = G K (projected)
+  Real dot(const DeviceVector& a, const DeviceVector& b) const;
+  // y = a x + b y
+  void axpby(Real a, const DeviceVector& x, Real b, DeviceVector& y) const;
+  // y = a x
+  void scale_copy(Real a, const DeviceVector& x, DeviceVector& y) const;
+  DeviceVector x_k;
+};
+
+class SolverCGEigen : public SolverEigenBase {
+ public:
+  using SolverEigenBase::SolverEigenBase;
+  std::string get_name() const override { return "CG"; }
+
+ protected:
+  bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) override;
+};
+
+class SolverBiCGSTABEigen : public SolverEigenBase {
+ public:
+  using SolverEigenBase::SolverEigenBase;
+  std::string get_name() const override { return "BiCGSTAB"; }
+
+ protected:
+  bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) override;
+};
+
+class SolverGMRESEigen : public SolverEigenBase {
+ public:
+  using SolverEigenBase::SolverEigenBase;
+  std::string get_name() const override { return "GMRES"; }
+
+ protected:
+  bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) override;
+  Uint restart{30};  // Eigen's GMRES default
+};
+
+class SolverDGMRESEigen : public SolverGMRESEigen {
+ public:
+  using SolverGMRESEigen::SolverGMRESEigen;
+  std::string get_name() const override { return "DGMRES"; }
+};
+
+class SolverMINRESEigen : public SolverEigenBase {
+ public:
+  using SolverEigenBase::SolverEigenBase;
+  std::string get_name() const override { return "MINRES"; }
+
+ protected:
+  bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) override;
+};
+
 // src/solver/solvers.hh:54-74
 // newton_maxiter: optional bound on the Newton iterations per load step
 // (no reference counterpart; 0 keeps the reference's bound, the solver's

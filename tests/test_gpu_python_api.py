@@ -139,9 +139,15 @@ def test_le34_homogeneous(µ, which, res, lengths):
     assert np.linalg.norm(r.stress.reshape(-1, 4) - stress.reshape(1, 4)) < 1e-8
 
 
-# ---- test_solver_newton_cg.cc:218-323: laminate patch test (Reuss) --------
+# ---- test_solver_newton_cg.cc:206-323: laminate patch test (Reuss), with
+# every solver of the reference's SolverList (:212-216) ---------------------
+SOLVERS = ["SolverCG", "SolverCGEigen", "SolverGMRESEigen", "SolverDGMRESEigen",
+           "SolverBiCGSTABEigen", "SolverMINRESEigen"]
+
+
+@pytest.mark.parametrize("solver_name", SOLVERS)
 @pytest.mark.parametrize("driver", ["newton_cg", "de_geus"])
-def test_laminate_patch_exact(µ, driver):
+def test_laminate_patch_exact(µ, driver, solver_name):
     sys = µ.Cell([3, 3], [1., 1.], µ.Formulation.small_strain)
     contrast, Young, Poisson = 2., 2., .33
     hard = µ.material.MaterialLinearElastic1_2d.make(sys, "hard", contrast * Young, Poisson)
@@ -153,7 +159,7 @@ def test_laminate_patch_exact(µ, driver):
     eps_soft = 1. / factor
     eps_hard = eps_soft / contrast
     for load, comps in [(np.array([[1., 0], [0, 0]]), [0]), (np.array([[0, 1.], [1., 0]]), [1, 2])]:
-        solver = µ.solvers.SolverCG(sys, 1e-8, 20, 0)
+        solver = getattr(µ.solvers, solver_name)(sys, 1e-8, 20, False)
         r = getattr(µ.solvers, driver)(sys, load, solver, 1e-5, 1e-10, 0)
         F = r.grad.reshape(-1, 4)
         for i, pixel in enumerate(sys):
@@ -308,3 +314,43 @@ def test_generic_material_equals_le1(µ, form):
     cell = µ.Cell(res, [1.] * 3, F)
     with pytest.raises(RuntimeError):
         µ.material.MaterialLinearElasticGeneric_3d.make(cell, "bad", np.eye(3))
+
+
+@pytest.mark.parametrize("solver_name", SOLVERS[1:])
+def test_eigen_solvers_small_case(µ, solver_name):
+    """bin/small_case.py:74-95 runs config 1 (51x51, disc of radius 10, LE1)
+    with every Eigen solver; each reaches the SolverCG solution to within
+    what two solves stopped at cg_tol = 1e-8 can differ by."""
+    res, lengths = [51, 51], [7., 5.]
+    mask = _disc(51, 10)
+    out = {}
+    for name in ("SolverCG", solver_name):
+        cell = µ.Cell(res, lengths, µ.Formulation.small_strain)
+        inc = µ.material.MaterialLinearElastic1_2d.make(cell, "incl", 10e9, .33)
+        mat = µ.material.MaterialLinearElastic1_2d.make(cell, "matrix", 70e9, .33)
+        for i, pixel in enumerate(cell):
+            (inc if mask.flat[i] else mat).add_pixel(pixel)
+        solver = getattr(µ.solvers, name)(cell, 1e-8, 401, False)
+        assert solver.name() == {"SolverCG": "CG", "SolverCGEigen": "CG", "SolverGMRESEigen": "GMRES",
+                                 "SolverDGMRESEigen": "DGMRES", "SolverBiCGSTABEigen": "BiCGSTAB",
+                                 "SolverMINRESEigen": "MINRES"}[name]
+        r = µ.solvers.newton_cg(cell, np.array([[0, 0], [0, .03]]), solver, 1e-5, 2, 0)
+        assert r.success and solver.get_counter() > 0
+        out[name] = np.asarray(r.grad).copy()
+    ref = out["SolverCG"]
+    assert np.linalg.norm(out[solver_name] - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("solver_name", SOLVERS[1:])
+def test_eigen_solver_convergence_error(µ, solver_name):
+    """SolverEigen::solve (solver_eigen.cc:69-77): too few iterations raise
+    ConvergenceError naming the solver."""
+    cell = µ.Cell([51, 51], [7., 5.], µ.Formulation.small_strain)
+    mask = _disc(51, 10)
+    inc = µ.material.MaterialLinearElastic1_2d.make(cell, "incl", 10e9, .33)
+    mat = µ.material.MaterialLinearElastic1_2d.make(cell, "matrix", 70e9, .33)
+    for i, pixel in enumerate(cell):
+        (inc if mask.flat[i] else mat).add_pixel(pixel)
+    solver = getattr(µ.solvers, solver_name)(cell, 1e-8, 2, False)
+    with pytest.raises(µ.ConvergenceError, match="has not converged"):
+        µ.solvers.newton_cg(cell, np.array([[0, 0], [0, .03]]), solver, 1e-5, 2, 0)
