@@ -222,7 +222,8 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
  * the host loop keeps the reference's control flow and reads back r.r once
  * per iteration (mugpu_cg_step) or once per chunk of iterations
  * (mugpu_cg_steps).
- * mugpu_cg_start: x = 0; r = A x - b (one full operator application);
+ * mugpu_cg_start: x = 0; r = A x - b (= 0 - b: the operator of a zero
+ *   field is zero exactly, so it is not applied; Ap is left unset);
  *   p = -r; returns rdr = r.r and rhs_norm2 = b.b.
  * mugpu_cg_step: [if update_p: p = -r + beta p with beta from the previous
  *   step] Ap = A p; alpha = rdr/(p.Ap); x += alpha p; r += alpha Ap;

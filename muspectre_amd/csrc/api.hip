@@ -1060,9 +1060,10 @@ extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* 
   }
   const int64_t n = c->N * c->nc;
   MUGPU_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, g_stream));
-  // r = A x - b, one full operator application (solver_cg.cc:62-63)
-  MUGPU_TRY(cell_operator64(c, d_x, d_Ap));
-  MUGPU_TRY(cg_init_vectors(MUGPU_FP64, d_Ap, d_rhs, d_r, d_p, n));
+  // r = A x - b (solver_cg.cc:62-63) at x = 0: the operator of a zero field
+  // is zero exactly (FFT, projection and K : p of zeros), so r = 0 - b
+  // without applying it
+  MUGPU_TRY(cg_init_vectors(MUGPU_FP64, nullptr, d_rhs, d_r, d_p, n));
   MUGPU_TRY(dot_device(MUGPU_FP64, d_r, d_r, n, c->partials, c->scal, FIN_STORE_RDR));
   MUGPU_TRY(dot_device(MUGPU_FP64, d_rhs, d_rhs, n, c->partials, c->scal, FIN_STORE_TMP0));
   MUGPU_TRY(sync_scalars(c->scal));

@@ -95,13 +95,13 @@ __global__ __launch_bounds__(kRT) void k_cg_update(T* __restrict__ x, T* __restr
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// r = Ax - b ; p = -r
+// r = Ax - b ; p = -r (Ax == nullptr: x = 0, where A x is zero exactly)
 template <typename T>
 __global__ __launch_bounds__(kRT) void k_cg_init(const T* __restrict__ Ax, const T* __restrict__ b,
                                                  T* __restrict__ r, T* __restrict__ p, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * kRT;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
-    const T rv = Ax[i] - b[i];
+    const T rv = (Ax ? Ax[i] : T(0)) - b[i];
     r[i] = rv;
     p[i] = -rv;
   }
