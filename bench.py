@@ -243,15 +243,23 @@ def kernel_bytes(wl, kclass, prec):
     kpts = int(np.prod(herm))
     if kclass in (1, 2):
         return 2 * kpts * nc * 2 * s
+    # with compile-time-length passes on the last axis the x update of a CG
+    # iteration rides on the next iteration's r2c (x += alpha p_old where p
+    # is read anyway), so the r2c moves x twice more and the update kernel
+    # reads neither x nor p
+    # (FP32 inner solves; FP64 with the isotropic (lambda, mu) tangent of
+    # small strain)
+    fused = wl.res[-1] in FAST and (prec == "fp32" or wl.form == "small")
     if kclass == 0:
-        # r and p read, p written (p = -r + beta p), tangent read, half
-        # spectrum written
-        return 3 * wl.N * nc * s + wl.N * wl.T_prec + kpts * nc * 2 * s
+        # r and p read, p written (p = -r + beta p), [x read and written],
+        # tangent read, half spectrum written
+        return (5 if fused else 3) * wl.N * nc * s + wl.N * wl.T_prec + kpts * nc * 2 * s
     if kclass == 3:
         # half spectrum read, Ap written, p read for the dot product
         return kpts * nc * 2 * s + 2 * wl.N * nc * s
     if kclass == 5:
-        return 6 * wl.N * nc * s
+        # [x, p read, x written,] r and Ap read, r written
+        return (3 if fused else 6) * wl.N * nc * s
     return None
 
 

@@ -221,7 +221,8 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
  * Device-resident CG.  The solver owns x, r, p, Ap (solver_cg.cc:45-51);
  * the host loop keeps the reference's control flow and reads back r.r once
  * per iteration (mugpu_cg_step) or once per chunk of iterations
- * (mugpu_cg_steps).
+ * (mugpu_cg_steps).  A cell runs one CG solve at a time: its device
+ * scalars carry the solve's r.r and beta from call to call.
  * mugpu_cg_start: x = 0; r = A x - b (= 0 - b: the operator of a zero
  *   field is zero exactly, so it is not applied; Ap is left unset);
  *   p = -r; returns rdr = r.r and rhs_norm2 = b.b.
@@ -235,7 +236,15 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
  *   new_rdr > tol2 (the first always runs).  *steps returns the iterations
  *   run (one whose r.r is NaN counts and stops the loop), *new_rdr the r.r
  *   of the last.  The host reads back once per call instead of once per
- *   iteration; the arithmetic is that of mugpu_cg_step.
+ *   iteration; the arithmetic is that of mugpu_cg_step.  Where the last
+ *   axis has compiled passes and the tangent is isotropic, the x update of
+ *   an iteration is applied by the next iteration's first pass (which reads
+ *   p anyway), so the x update of the last iteration of a call is left
+ *   pending (its alpha kept by the cell): the next call with update_p = 1
+ *   on the same x and p applies it, and mugpu_cg_finish applies it after
+ *   the last call; mugpu_cg_step, mugpu_cg_start and a call on other
+ *   vectors apply it first.  x and p must stay allocated until then.
+ * mugpu_cg_finish: applies a pending x update (no-op otherwise).
  * The operator runs in FP64 (cells created with MUGPU_FP32 solve with
  * mugpu_cg_solve_mixed). */
 int mugpu_cg_start(mugpu_cell cell, const void *d_rhs, void *d_x, void *d_r,
@@ -245,6 +254,7 @@ int mugpu_cg_step(mugpu_cell cell, int update_p, void *d_x, void *d_r,
 int mugpu_cg_steps(mugpu_cell cell, int update_p, void *d_x, void *d_r,
                    void *d_p, void *d_Ap, double tol2, int max_steps,
                    int *steps, double *new_rdr);
+int mugpu_cg_finish(mugpu_cell cell);
 
 /* ---- mixed-precision CG with FP64 refinement (no reference counterpart;
  * it replaces SolverCG::solve for cells created with MUGPU_FP32) --------

@@ -83,6 +83,7 @@ SIGNATURES = [
     ("mugpu_cg_start", _int, [_vp, _vp, _vp, _vp, _vp, _vp, _dp, _dp]),
     ("mugpu_cg_step", _int, [_vp, _int, _vp, _vp, _vp, _vp, _dp]),
     ("mugpu_cg_steps", _int, [_vp, _int, _vp, _vp, _vp, _vp, _dbl, _int, _ip, _dp]),
+    ("mugpu_cg_finish", _int, [_vp]),
     ("mugpu_cg_solve_mixed", _int, [_vp, _vp, _vp, _dbl, _int, _dbl, _i64p, _ip, _dp]),
     ("mugpu_timing_start", _int, [_int]),
     ("mugpu_timing_stop", _int, [_i64p, _dp]),

@@ -122,6 +122,10 @@ struct mugpu_cell_s {
   float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr;
   double* r64 = nullptr;
   int* ctl = nullptr;  // CTL_* words of the device-resident inner CG loop
+  // mugpu_cg_steps left the x update of its last iteration to be applied
+  // (to x_pend with p_pend) by the next call or by mugpu_cg_finish
+  bool x_pending = false;
+  void *x_pend = nullptr, *p_pend = nullptr;
 };
 
 namespace {
@@ -239,10 +243,13 @@ int ensure_ctl(mugpu_cell c) {
 
 // one iteration of the FP32 inner CG (solver_cg.cc:62-96): A p fused with
 // the p update (all but the first iteration), p.Ap, the x and r updates and
-// r.r, gated by the cell's control words
-int inner_iteration(mugpu_cell c, bool first, int64_t n) {
+// r.r, gated by the cell's control words.  fuse: the previous iteration's
+// x update rides on this iteration's r2c (x += alpha p_old on the way in)
+// instead of the CG update kernel, which then reads neither x nor p
+int inner_iteration(mugpu_cell c, bool first, bool fuse, int64_t n) {
   OpSpec op;
   op.in_mode = first ? IN_TAN : IN_TAN_PUPDATE;
+  op.xvec = fuse && !first ? c->x32 : nullptr;
   op.in = c->p32;
   op.rvec = c->r32;
   op.pout = c->p32;
@@ -254,7 +261,8 @@ int inner_iteration(mugpu_cell c, bool first, int64_t n) {
   op.dotv = c->p32;
   MUGPU_TRY(run_operator(c->plan, MUGPU_FP32, op, c->pj, c->spectrum));
   MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzb[1], c->scal, FIN_ALPHA, c->ctl));
-  return cg_update(MUGPU_FP32, c->x32, c->r32, c->p32, c->Ap32, c->scal, n, c->partials, c->ctl);
+  return cg_update(MUGPU_FP32, fuse ? nullptr : c->x32, c->r32, c->p32, c->Ap32, c->scal, n,
+                   c->partials, c->ctl);
 }
 
 }  // namespace
@@ -1051,6 +1059,8 @@ extern "C" int mugpu_scale_copy(double a, const void* d_x, void* d_y, int64_t n)
 }
 
 // ============================== SolverCG ======================================
+int flush_x(mugpu_cell c);
+
 extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* d_r, void* d_p,
                               void* d_Ap, double* rdr, double* rhs_norm2) {
   if (!c) return MUGPU_EARG;
@@ -1059,6 +1069,7 @@ extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* 
     return MUGPU_ERUNTIME;
   }
   const int64_t n = c->N * c->nc;
+  MUGPU_TRY(flush_x(c));
   MUGPU_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, g_stream));
   // r = A x - b (solver_cg.cc:62-63) at x = 0: the operator of a zero field
   // is zero exactly (FFT, projection and K : p of zeros), so r = 0 - b
@@ -1072,9 +1083,22 @@ extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* 
   return MUGPU_OK;
 }
 
+// the x update left by mugpu_cg_steps: x += alpha p (alpha of its last iteration)
+int flush_x(mugpu_cell c) {
+  if (!c->x_pending) return MUGPU_OK;
+  c->x_pending = false;
+  return axpy_alpha(MUGPU_FP64, c->x_pend, c->p_pend, c->scal, c->N * c->nc, SC_XALPHA);
+}
+
+extern "C" int mugpu_cg_finish(mugpu_cell c) {
+  if (!c) return MUGPU_EARG;
+  return flush_x(c);
+}
+
 extern "C" int mugpu_cg_step(mugpu_cell c, int update_p, void* d_x, void* d_r, void* d_p,
                              void* d_Ap, double* new_rdr) {
   if (!c) return MUGPU_EARG;
+  MUGPU_TRY(flush_x(c));
   const int64_t n = c->N * c->nc;
   OpSpec op;
   op.in_mode = update_p ? IN_TAN_PUPDATE : IN_TAN;
@@ -1116,18 +1140,39 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
   // remaining kernels return at once
   int rc = MUGPU_OK;
   std::vector<int64_t> marks;
+  // the x update of iteration i rides on the r2c of iteration i + 1 (x +=
+  // alpha p_old on the way in) where the last axis has compiled passes and
+  // that r2c stages p through LDS (isotropic tangent; with a 45/81-entry
+  // FP64 tangent it loads per voxel, and x would double its strided
+  // traffic); the last one is left pending for the next call or
+  // mugpu_cg_finish
+  const bool fuse = c->plan.fastax[c->dim - 1] && c->tan_kind == TAN_ISO;
+  const bool pend_here = c->x_pending && c->x_pend == d_x && c->p_pend == d_p && update_p;
+  if (c->x_pending && !pend_here) MUGPU_TRY(flush_x(c));
+  // the first r2c applies the pending update with scal[SC_ALPHA]
+  if (pend_here) MUGPU_TRY(scal_copy(c->scal, SC_XALPHA, SC_ALPHA));
+  c->x_pending = false;
   c->plan.active = c->ctl;
   for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) {
     marks.push_back(timing_mark());
     op.in_mode = (i > 0 || update_p) ? IN_TAN_PUPDATE : IN_TAN;
+    op.xvec = fuse && (i > 0 || pend_here) ? d_x : nullptr;
     rc = run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
     if (rc == MUGPU_OK)
       rc = finalize((const double*)c->plan.partials, c->plan.nzb[0], c->scal, FIN_ALPHA, c->ctl);
     if (rc == MUGPU_OK)
-      rc = cg_update(MUGPU_FP64, d_x, d_r, d_p, d_Ap, c->scal, n, c->partials, c->ctl);
+      rc = cg_update(MUGPU_FP64, fuse ? nullptr : d_x, d_r, d_p, d_Ap, c->scal, n, c->partials,
+                     c->ctl);
   }
   c->plan.active = nullptr;
   MUGPU_TRY(rc);
+  if (fuse) {
+    // alpha of the last iteration run (launches after the stop leave it)
+    MUGPU_TRY(scal_copy(c->scal, SC_ALPHA, SC_XALPHA));
+    c->x_pending = true;
+    c->x_pend = d_x;
+    c->p_pend = d_p;
+  }
   MUGPU_TRY(sync_ctl(c->ctl, c->scal));
   const double r = g_host_scal[SC_NEWRDR];
   // a NaN r.r stops the device loop without counting its iteration; the
@@ -1206,11 +1251,13 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     int launched = 0;
     int rc = MUGPU_OK;
     std::vector<int64_t> marks;
+    // x updates fused into the next r2c (compile-time-length last axis)
+    const bool fuse = c->plan.fastax[c->dim - 1];
     c->plan.active = c->ctl;
     while (rc == MUGPU_OK) {
       for (int j = 0; j < kChunk && launched < maxit && rc == MUGPU_OK; ++j, ++launched) {
         marks.push_back(timing_mark());
-        rc = inner_iteration(c, launched == 0, n);
+        rc = inner_iteration(c, launched == 0, fuse, n);
       }
       if (rc != MUGPU_OK) break;
       rc = sync_ctl(c->ctl, c->scal);
@@ -1225,6 +1272,8 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
       if (ran < (int)marks.size()) timing_void(marks[ran]);
     }
     its += g_host_ctl[CTL_ITER];
+    // the last iteration's x update (alpha and p of the last iteration run)
+    if (fuse) MUGPU_TRY(axpy_alpha(MUGPU_FP32, c->x32, c->p32, c->scal, n));
     MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n));
   }
 

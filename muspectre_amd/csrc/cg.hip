@@ -86,13 +86,23 @@ __global__ __launch_bounds__(kRT) void k_cg_update(T* __restrict__ x, T* __restr
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kRT;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
-    x[i] += al * p[i];
+    if (x) x[i] += al * p[i];
     const T rv = r[i] + al * Ap[i];
     r[i] = rv;
     acc += (double)rv * (double)rv;
   }
   const double t = block_reduce<kRT>(acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// x += alpha p, alpha = scal[SC_ALPHA]
+template <typename T>
+__global__ __launch_bounds__(kRT) void k_axpy_alpha(T* __restrict__ x, const T* __restrict__ p,
+                                                    const double* __restrict__ scal, int64_t n,
+                                                    int slot) {
+  const T al = (T)scal[slot];
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) x[i] += al * p[i];
 }
 
 // r = Ax - b ; p = -r (Ax == nullptr: x = 0, where A x is zero exactly)
@@ -146,11 +156,18 @@ __global__ __launch_bounds__(kRT) void k_fill(double* __restrict__ F, Uniform u,
   }
 }
 
-inline unsigned grid_for(int64_t n) {
+// grid of the grid-stride kernels: at most `cap` workgroups (kRedGrid,
+// measured best for kernels that read 16 B or more per value; the FP32 CG
+// update without x reads 8 B per value and runs best with kRedBlocks)
+inline unsigned grid_for(int64_t n, int cap = kRedGrid) {
   int64_t g = (n + kRT - 1) / kRT;
-  if (g > kRedBlocks) g = kRedBlocks;
+  if (g > cap) g = cap;
   if (g < 1) g = 1;
   return (unsigned)g;
+}
+
+__global__ void k_scal_copy(double* scal, int from, int to) {
+  if (threadIdx.x == 0) scal[to] = scal[from];
 }
 
 // control words of a device-resident CG loop and its r.r threshold
@@ -296,7 +313,7 @@ int finalize(const double* d_partials, int64_t count, double* d_scal, int mode, 
 
 int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
               int64_t n, double* d_partials, int* ctl) {
-  const unsigned g = grid_for(n);
+  const unsigned g = grid_for(n, prec == MUGPU_FP32 && !x ? kRedBlocks : kRedGrid);
   timing_begin(MUGPU_K_CG_UPDATE);
   if (prec == MUGPU_FP64)
     hipLaunchKernelGGL((k_cg_update<double>), dim3(g), dim3(kRT), 0, stream(), (double*)x,
@@ -307,6 +324,25 @@ int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const d
   timing_end(MUGPU_K_CG_UPDATE);
   MUGPU_HIP(hipGetLastError());
   return finalize(d_partials, g, const_cast<double*>(d_scal), FIN_BETA, ctl);
+}
+
+int axpy_alpha(int prec, void* x, const void* p, const double* d_scal, int64_t n, int slot) {
+  const unsigned g = grid_for(n);
+  if (prec == MUGPU_FP64)
+    hipLaunchKernelGGL((k_axpy_alpha<double>), dim3(g), dim3(kRT), 0, stream(), (double*)x,
+                       (const double*)p, d_scal, n, slot);
+  else
+    hipLaunchKernelGGL((k_axpy_alpha<float>), dim3(g), dim3(kRT), 0, stream(), (float*)x,
+                       (const float*)p, d_scal, n, slot);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int scal_copy(double* d_scal, int from, int to) {
+  if (from < 0 || from >= SC_COUNT || to < 0 || to >= SC_COUNT) return MUGPU_EARG;
+  hipLaunchKernelGGL(k_scal_copy, dim3(1), dim3(64), 0, stream(), d_scal, from, to);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
 }
 
 int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n) {

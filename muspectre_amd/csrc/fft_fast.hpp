@@ -374,6 +374,7 @@ struct LArgs {
   int64_t rows;
   const c_t<T>* tw;
   const int* active;  // FftPlan::active (single-cell kernels only)
+  T* x;               // OpSpec::xvec (IN_TAN_PUPDATE; nullptr: no x update)
 };
 
 // ISO: the tangent is known to be isotropic (TAN_ISO), which the compiler
@@ -401,17 +402,21 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
     // per CU, and the tangent loads of a voxel are better issued together
     // with its p and r loads than after a barrier.)
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
+    // the previous iteration's x update, x += alpha p_old, when fused here
+    const bool xupd = IN == IN_TAN_PUPDATE && a.x != nullptr;
+    const T al = xupd ? (T)a.scal[SC_ALPHA] : T(0);
     {
       constexpr int TOT = RB * N * NC, kBatch = 8;
       const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
       for (int e0 = 0; e0 < TOT; e0 += kBatch * NT) {
-        T pv[kBatch], rv[kBatch];
+        T pv[kBatch], rv[kBatch], xv[kBatch];
 #pragma unroll
         for (int i = 0; i < kBatch; ++i) {
           const int e = e0 + i * NT + threadIdx.x;
           const int64_t g = base + e < lim ? base + e : lim - 1;
           pv[i] = a.in[g];
           if (IN == IN_TAN_PUPDATE) rv[i] = a.rvec[g];
+          if (IN == IN_TAN_PUPDATE && xupd) xv[i] = a.x[g];
         }
 #pragma unroll
         for (int i = 0; i < kBatch; ++i) {
@@ -421,6 +426,7 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
             T x = T(0);
             if (valid) {
               if (IN == IN_TAN_PUPDATE) {
+                if (xupd) a.x[base + e] = xv[i] + al * pv[i];
                 x = -rv[i] + beta * pv[i];
                 a.pout[base + e] = x;
               } else {
@@ -452,6 +458,7 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   } else {
     // one voxel per thread and step: p, r and tangent loads issued together
     // (FP64, tangent kind chosen at run time)
+    // (no fused x update here: the launcher refuses OpSpec::xvec for it)
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
     for (int e = threadIdx.x; e < RB * N; e += NT) {
       const int r = e / N, m = e - (e / N) * N;

@@ -19,11 +19,14 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.rows = p.N / N;
   a.tw = tw_of<T>(p, DIM - 1);
   a.active = p.active;
+  a.x = reinterpret_cast<T*>(op.xvec);
   constexpr int RB = z_rows(N, DIM, H);
   constexpr int NT = RB / 2 * DIM * DIM * (N / E);
   constexpr size_t lds = lds_lines<T, N, DIM, H>();
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   const bool iso = op.tan_kind == TAN_ISO;
+  // the fused x update exists only where p is staged through LDS
+  if (op.xvec && (op.in_mode != IN_TAN_PUPDATE || (sizeof(T) == 8 && !iso))) return MUGPU_EARG;
   if (op.in_mode == IN_PLAIN)
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN, false, H>), grid, dim3(NT), lds, stream(), a);
   else if (op.in_mode == IN_TAN && iso)

@@ -717,6 +717,8 @@ int launch_project(FftPlan& p, const ProjParams& pj, c_t<T>* spec) {
 template <typename T, int DIM>
 int operator_impl(FftPlan& p, const OpSpec& op, const ProjParams& pj, void* spectrum) {
   c_t<T>* spec = reinterpret_cast<c_t<T>*>(spectrum);
+  // the fused x update is carried by the compiled last-axis r2c only
+  if (op.xvec && !p.fastax[DIM - 1]) return MUGPU_EARG;
   MUGPU_TRY((launch_r2c<T, DIM>(p, op, spec)));
   if (DIM == 3) MUGPU_TRY(launch_strided<T>(p, 1, p.yW, spec, -1));
   if (pj.kind >= 0) {

@@ -499,6 +499,8 @@ DeviceVector& SolverCG::solve(const DeviceVector& rhs) {
                 << ": |r|/|b| = " << std::setw(15) << std::sqrt(rdr / rhs_norm2)
                 << ", cg_tol = " << tol << std::endl;
   }
+  // the x update of the last iteration (left pending by mugpu_cg_steps)
+  check(mugpu_cg_finish(h));
   if (rdr < tol2) {
     converged = true;
   } else {

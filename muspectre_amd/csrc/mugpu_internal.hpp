@@ -79,7 +79,10 @@ __host__ __device__ constexpr int sym_index(int r, int c, int n) {
 
 // device scalar block used by the CG kernels
 // SC_TOL2: squared stopping threshold of a batched inner CG
-enum Scalar { SC_RDR = 0, SC_PAP, SC_ALPHA, SC_NEWRDR, SC_BETA, SC_TMP0, SC_TMP1, SC_TOL2, SC_COUNT };
+// SC_XALPHA: alpha of a pending x update (kept apart from SC_ALPHA, which
+// the next solve on the cell overwrites)
+enum Scalar { SC_RDR = 0, SC_PAP, SC_ALPHA, SC_NEWRDR, SC_BETA, SC_TMP0, SC_TMP1, SC_TOL2,
+              SC_XALPHA, SC_COUNT };
 
 // ---- FFT plan -----------------------------------------------------------------
 struct FftPlan {
@@ -139,6 +142,10 @@ struct OpSpec {
   const void* in = nullptr;      // real field (plan precision)
   const void* rvec = nullptr;    // CG residual for IN_TAN_PUPDATE
   void* pout = nullptr;          // updated p for IN_TAN_PUPDATE
+  // IN_TAN_PUPDATE with compile-time-length last axis only: x += alpha p_old
+  // on the way in (alpha = scal[SC_ALPHA], the previous iteration's), so
+  // that the CG update need not read p again
+  void* xvec = nullptr;
   const double* scal = nullptr;  // device scalars (SC_BETA)
   int tan_kind = TAN_FULL;
   const void* tangent = nullptr; // plan precision
@@ -211,14 +218,22 @@ int finalize(const double* d_partials, int64_t count, double* d_scal, int mode,
 int set_ctl(int* ctl, int maxit, double* d_scal, double tol2);
 enum FinalizeMode { FIN_ALPHA = 0, FIN_BETA = 1, FIN_STORE_TMP0 = 2, FIN_STORE_TMP1 = 3,
                     FIN_STORE_RDR = 4 };
+// x == nullptr: x is updated elsewhere (OpSpec::xvec of the next r2c)
 int cg_update(int prec, void* x, void* r, const void* p, const void* Ap, const double* d_scal,
               int64_t n, double* d_partials, int* ctl = nullptr);
+// x += alpha p with alpha = scal[slot] (the update left over when the CG
+// x update rides on the next iteration's r2c)
+int axpy_alpha(int prec, void* x, const void* p, const double* d_scal, int64_t n,
+               int slot = SC_ALPHA);
+// scal[to] = scal[from] on the device
+int scal_copy(double* d_scal, int from, int to);
 int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n);
 int axpby(int prec, double a, const void* x, double b, void* y, int64_t n);
 int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, double scale);
 int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n);
 int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add);
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = 4096;  // capacity of the partial-sum buffers
+constexpr int kRedGrid = 1024;    // default workgroup cap of the grid-stride kernels
 // workgroups (= partial sums) of the reductions over n values
 int reduction_blocks(int64_t n);
 

@@ -211,6 +211,7 @@ def _gpu_cg_chunked(gc, rhs, tol, maxiter, chunk):
         rd = nr.value
         it += done.value
         calls += 1
+    capi.check(lib.mugpu_cg_finish(gc.h))
     return x.download((gc.N, gc.nc)), r.download((gc.N, gc.nc)), it, rd, calls
 
 
@@ -246,6 +247,52 @@ def test_cg_steps_equal_step_loop(capi, case, chunk):
         assert calls <= -(-it1 // chunk) + 1
         np.testing.assert_array_equal(x2, x1)
     assert conv1 is False and it1 == 7
+
+
+@pytest.mark.parametrize("case", [CASES[6], CASES[9], CASES[1]], ids=lambda c: c[0])
+def test_cg_steps_pending_update_then_new_solve(capi, case):
+    """A solve stopped after some chunks without mugpu_cg_finish: the next
+    solve's mugpu_cg_start applies its pending x update with its own alpha
+    (kept apart from the scalars the new solve overwrites), so its x is that
+    of a host loop run for as many iterations, and the new solve is
+    unaffected."""
+    name, res, L, form, f_in, f_out, Ein, Eout = case[:8]
+    even = case[8] if len(case) > 8 else 0
+    gc = GpuCell(capi, res, L, form, allow_even=even)
+    mask = _sphere_mask(res, min(res) / 3)
+    mi = gc.add_material(f_in, Ein, .3)
+    mo = gc.add_material(f_out, Eout, .3)
+    idx = np.arange(mask.size)
+    gc.add_pixels(mi, idx[mask.ravel()])
+    gc.add_pixels(mo, idx[~mask.ravel()])
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    F = np.zeros((gc.N, gc.nc))
+    F[:, 1] += 0.01
+    gc.set_F(F)
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    lib = gc.lib
+    n = gc.N * gc.nc
+    rng = np.random.default_rng(11)
+    rhs_a, rhs_b = rng.standard_normal((gc.N, gc.nc)), rng.standard_normal((gc.N, gc.nc))
+    ref_a, it_a, _ = _gpu_cg(gc, rhs_a, 1e-30, 7)
+    assert it_a == 7
+    ref_b = _gpu_cg(gc, rhs_b, 1e-8, 200)[0]
+    b = DeviceBuffer.from_array(rhs_a)
+    x, r, p, Ap = (DeviceBuffer(8 * n) for _ in range(4))
+    rdr, bn2 = ctypes.c_double(), ctypes.c_double()
+    capi.check(lib.mugpu_cg_start(gc.h, b.ptr, x.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(rdr),
+                                  ctypes.byref(bn2)))
+    it = 0
+    for k in (3, 4):
+        nr, done = ctypes.c_double(), ctypes.c_int()
+        capi.check(lib.mugpu_cg_steps(gc.h, 1 if it > 0 else 0, x.ptr, r.ptr, p.ptr, Ap.ptr, 0.0, k,
+                                      ctypes.byref(done), ctypes.byref(nr)))
+        assert done.value == k
+        it += k
+    # no mugpu_cg_finish: the next solve's start applies the pending update
+    x_b, _, _, _, _ = _gpu_cg_chunked(gc, rhs_b, 1e-8, 200, 8)
+    np.testing.assert_array_equal(x.download((gc.N, gc.nc)), ref_a)
+    np.testing.assert_array_equal(x_b, ref_b)
 
 
 def _aniso_voigt(dim, seed, major_sym=True):
