@@ -82,9 +82,17 @@ const char *mugpu_last_error(void);
 int mugpu_device_info(char *name, int name_len, int *cu_count,
                       size_t *hbm_bytes, int *lds_per_block);
 /* field memory: in the reference every field is a std::vector<Real> owned
- * by a FieldCollection (src/common/field_typed.hh:443-449) */
+ * by a FieldCollection (src/common/field_typed.hh:443-449).  Blocks are
+ * cached: mugpu_free keeps a block (up to a quarter of the device memory in
+ * all) for the next mugpu_alloc of the same size, so repeated solves do not
+ * map and unmap memory each time; the blocks are for work on the library
+ * stream.  mugpu_free of a pointer mugpu_alloc did not return (or already
+ * freed) fails with MUGPU_EARG.  mugpu_release_cached (no reference
+ * counterpart) returns the cached blocks to the device after the library
+ * stream's work; an allocation that fails does so by itself and retries. */
 int mugpu_alloc(size_t bytes, void **d_ptr);
 int mugpu_free(void *d_ptr);
+int mugpu_release_cached(void);
 int mugpu_upload(void *d_dst, const void *h_src, size_t bytes);
 int mugpu_download(void *h_dst, const void *d_src, size_t bytes);
 int mugpu_memset_zero(void *d_ptr, size_t bytes);

@@ -41,6 +41,7 @@ SIGNATURES = [
                                  ctypes.POINTER(ctypes.c_size_t), _ip]),
     ("mugpu_alloc", _int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
     ("mugpu_free", _int, [_vp]),
+    ("mugpu_release_cached", _int, []),
     ("mugpu_upload", _int, [_vp, _vp, ctypes.c_size_t]),
     ("mugpu_download", _int, [_vp, _vp, ctypes.c_size_t]),
     ("mugpu_memset_zero", _int, [_vp, ctypes.c_size_t]),

@@ -5,8 +5,11 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "mugpu_internal.hpp"
@@ -305,20 +308,92 @@ extern "C" int mugpu_device_info(char* name, int name_len, int* cu_count, size_t
   return MUGPU_OK;
 }
 
+// Field memory is cached: mugpu_free keeps a block for the next mugpu_alloc
+// of the same (rounded) size, so repeated solves (a new solver, work and
+// result fields per call) do not pay hipMalloc, which maps the pages, and
+// hipFree, which also waits for the device, every time.  Every user of these
+// blocks works on the library stream or copies synchronously, so stream
+// order makes reuse safe.  The cache holds at most a quarter of the device
+// memory and is emptied when an allocation fails (then retried) and by
+// mugpu_release_cached.
+namespace {
+std::mutex g_alloc_mu;
+std::multimap<size_t, void*> g_cached;      // size -> free block
+std::unordered_map<void*, size_t> g_blocks;  // live block -> size
+size_t g_cached_bytes = 0, g_cache_cap = 0;
+
+size_t round_block(size_t bytes) {
+  constexpr size_t kSmall = 512, kLarge = size_t(2) << 20;
+  if (bytes == 0) bytes = 16;
+  const size_t g = bytes >= (size_t(1) << 20) ? kLarge : kSmall;
+  return (bytes + g - 1) / g * g;
+}
+
+void release_cached_locked() {
+  for (auto& kv : g_cached) (void)hipFree(kv.second);
+  g_cached.clear();
+  g_cached_bytes = 0;
+}
+}  // namespace
+
 extern "C" int mugpu_alloc(size_t bytes, void** d_ptr) {
   MUGPU_TRY(ensure_init());
   *d_ptr = nullptr;
-  if (bytes == 0) bytes = 16;
-  hipError_t e = hipMalloc(d_ptr, bytes);
+  const size_t sz = round_block(bytes);
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  if (g_cache_cap == 0) {
+    size_t fr = 0, tot = 0;
+    MUGPU_HIP(hipMemGetInfo(&fr, &tot));
+    g_cache_cap = tot / 4;
+  }
+  auto it = g_cached.find(sz);
+  if (it != g_cached.end()) {
+    *d_ptr = it->second;
+    g_cached.erase(it);
+    g_cached_bytes -= sz;
+    g_blocks[*d_ptr] = sz;
+    return MUGPU_OK;
+  }
+  hipError_t e = hipMalloc(d_ptr, sz);
+  if (e != hipSuccess && !g_cached.empty()) {
+    (void)hipGetLastError();
+    release_cached_locked();
+    e = hipMalloc(d_ptr, sz);
+  }
   if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *d_ptr = nullptr;
     set_error("device allocation of %zu bytes failed: %s", bytes, hipGetErrorString(e));
     return MUGPU_EMEMORY;
   }
+  g_blocks[*d_ptr] = sz;
   return MUGPU_OK;
 }
 
 extern "C" int mugpu_free(void* d_ptr) {
-  if (d_ptr) MUGPU_HIP(hipFree(d_ptr));
+  if (!d_ptr) return MUGPU_OK;
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  auto it = g_blocks.find(d_ptr);
+  if (it == g_blocks.end()) {
+    set_error("mugpu_free: %p was not allocated by mugpu_alloc (or was freed already)", d_ptr);
+    return MUGPU_EARG;
+  }
+  const size_t sz = it->second;
+  g_blocks.erase(it);
+  if (g_cached_bytes + sz <= g_cache_cap) {
+    g_cached.emplace(sz, d_ptr);
+    g_cached_bytes += sz;
+    return MUGPU_OK;
+  }
+  MUGPU_HIP(hipFree(d_ptr));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_release_cached(void) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  if (g_cached.empty()) return MUGPU_OK;
+  MUGPU_HIP(hipStreamSynchronize(g_stream));
+  release_cached_locked();
   return MUGPU_OK;
 }
 

@@ -460,30 +460,50 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
     // (FP64, tangent kind chosen at run time)
     // (no fused x update here: the launcher refuses OpSpec::xvec for it)
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
-    for (int e = threadIdx.x; e < RB * N; e += NT) {
-      const int r = e / N, m = e - (e / N) * N;
-      const int64_t row = row0 + r;
-      T dp[NC];
-      if (row < a.rows) {
-        const int64_t vox = row * N + m;
-        T pv[NC];
-        if (IN == IN_TAN_PUPDATE) {
+    // one loop per tangent kind, the kind a compile-time constant in each:
+    // a run-time switch inside the loop keeps the other kinds' values live
+    // and leaves too few registers for the tangent loads, which then go out
+    // one at a time
+    auto voxels = [&](auto kind_c) {
+      constexpr int KIND = decltype(kind_c)::value;
+      for (int e = threadIdx.x; e < RB * N; e += NT) {
+        const int r = e / N, m = e - (e / N) * N;
+        const int64_t row = row0 + r;
+        T dp[NC];
+        if (row < a.rows) {
+          const int64_t vox = row * N + m;
+          T pv[NC];
+          if (IN == IN_TAN_PUPDATE) {
+            // all loads before the first store (p is updated in place)
+            T rv[NC];
 #pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            pv[c] = -a.rvec[vox * NC + c] + beta * a.in[vox * NC + c];
-            a.pout[vox * NC + c] = pv[c];
+            for (int c = 0; c < NC; ++c) {
+              rv[c] = a.rvec[vox * NC + c];
+              pv[c] = a.in[vox * NC + c];
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              pv[c] = -rv[c] + beta * pv[c];
+              a.pout[vox * NC + c] = pv[c];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) pv[c] = a.in[vox * NC + c];
           }
+          tangent_product<T, DIM>(KIND, a.tan, vox, pv, dp);
         } else {
 #pragma unroll
-          for (int c = 0; c < NC; ++c) pv[c] = a.in[vox * NC + c];
+          for (int c = 0; c < NC; ++c) dp[c] = T(0);
         }
-        tangent_product<T, DIM>(ISO ? TAN_ISO : a.tan_kind, a.tan, vox, pv, dp);
-      } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dp[c] = T(0);
+        for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
       }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
+    };
+    switch (a.tan_kind) {
+      case TAN_SYM: voxels(std::integral_constant<int, TAN_SYM>{}); break;
+      case TAN_HEP: voxels(std::integral_constant<int, TAN_HEP>{}); break;
+      case TAN_ISO: voxels(std::integral_constant<int, TAN_ISO>{}); break;
+      default: voxels(std::integral_constant<int, TAN_FULL>{}); break;
     }
   }
   __syncthreads();

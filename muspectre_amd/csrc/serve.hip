@@ -762,6 +762,9 @@ extern "C" int mugpu_server_create(const mugpu_server_config* cfg, mugpu_server*
   }
   auto s = std::make_unique<mugpu_server_s>();
   s->cfg = *cfg;
+  // field blocks cached by mugpu_free go back to the device first: the
+  // arena is sized from (and taken out of) free memory
+  MUGPU_TRY(mugpu_release_cached());
   size_t budget = cfg->hbm_budget;
   if (budget == 0) {
     size_t fr = 0, tot = 0;

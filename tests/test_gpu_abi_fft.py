@@ -146,3 +146,34 @@ def test_projection_even_policy_vs_oracle(capi, oracle, res, kind):
         assert np.abs(out2 - out).max() <= 1e-13 * np.abs(out).max()
     finally:
         lib.mugpu_projection_destroy(p)
+
+
+def test_field_memory_cache(capi):
+    """mugpu_alloc / mugpu_free keep freed blocks for the next allocation of
+    the same size; contents survive an upload/download round trip in a
+    reused block; a second free and a foreign pointer are refused;
+    mugpu_release_cached returns the blocks to the device."""
+    lib = capi.load()
+    nb = (3 << 20) + 5
+    capi.check(lib.mugpu_release_cached())  # blocks of earlier tests
+    a = ctypes.c_void_p()
+    capi.check(lib.mugpu_alloc(nb, ctypes.byref(a)))
+    capi.check(lib.mugpu_free(a))
+    b = ctypes.c_void_p()
+    capi.check(lib.mugpu_alloc(nb - 3, ctypes.byref(b)))  # same rounded size
+    assert b.value == a.value
+    x = np.random.default_rng(1).standard_normal((nb - 3) // 8)
+    capi.check(lib.mugpu_upload(b, x.ctypes.data_as(ctypes.c_void_p), x.nbytes))
+    y = np.empty_like(x)
+    capi.check(lib.mugpu_download(y.ctypes.data_as(ctypes.c_void_p), b, y.nbytes))
+    np.testing.assert_array_equal(y, x)
+    capi.check(lib.mugpu_free(b))
+    assert lib.mugpu_free(b) == 7  # MUGPU_EARG: freed already
+    c = ctypes.c_void_p()
+    capi.check(lib.mugpu_alloc(nb, ctypes.byref(c)))
+    assert lib.mugpu_free(ctypes.c_void_p(c.value + 256)) == 7  # not a block start
+    capi.check(lib.mugpu_free(c))
+    capi.check(lib.mugpu_release_cached())
+    d = ctypes.c_void_p()
+    capi.check(lib.mugpu_alloc(16, ctypes.byref(d)))
+    capi.check(lib.mugpu_free(d))
