@@ -310,9 +310,9 @@ def cpu_baseline(wl, budget_s):
         if el > budget_s or iters >= 200:
             break
     return {"value": N * iters / el, "unit": "voxel-CG-iters/s", "cores": 1, "kind": "port",
-            "sample": "{} {} voxels, {} CG iterations of the oracle (FP64, stored 9x9 "
+            "sample": "{} {} voxels, {} CG iterations of the oracle (FP64, stored {}x{} "
                       "tangent and projection operator as in the reference), {:.1f} s"
-                      .format(wl.name, "x".join(map(str, res)), iters, el)}
+                      .format(wl.name, "x".join(map(str, res)), iters, dim * dim, dim * dim, el)}
 
 
 def copy_bandwidth(capi, nbytes=4 << 30, reps=5):

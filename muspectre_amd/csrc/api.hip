@@ -1291,15 +1291,16 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
   bool first = true;
   while (true) {
     // true residual with the FP64 operator: r64 = b - A x; at the start
-    // x = 0, so A x = 0 exactly and r64 = b, |r64|^2 = |b|^2 bit for bit
+    // x = 0, so A x = 0 exactly and the residual is b itself (read in place
+    // of r64 below), |b|^2 bit for bit
+    const double* res64 = first ? (const double*)d_rhs : c->r64;
     if (first) {
-      MUGPU_TRY(convert(MUGPU_FP64, d_rhs, MUGPU_FP64, c->r64, n, 1.0));
       rr = bnorm2;
       first = false;
     } else {
       MUGPU_TRY(cell_operator64(c, d_x, c->r64));
-      MUGPU_TRY(axpby(MUGPU_FP64, 1.0, d_rhs, -1.0, c->r64, n));
-      MUGPU_TRY(dot_device(MUGPU_FP64, c->r64, c->r64, n, c->partials, c->scal, FIN_STORE_TMP1));
+      MUGPU_TRY(residual_dot((const double*)d_rhs, c->r64, n, c->partials, c->scal,
+                             FIN_STORE_TMP1));
       MUGPU_TRY(sync_scalars(c->scal));
       rr = g_host_scal[SC_TMP1];
     }
@@ -1307,10 +1308,9 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     if (!(rr > tol2) || its >= maxiter || !(rr == rr)) break;
     // inner CG in FP32 on A d = r64 / |r64|
     const double s = 1.0 / std::sqrt(rr);
-    MUGPU_HIP(hipMemsetAsync(c->x32, 0, sizeof(float) * n, g_stream));
-    MUGPU_TRY(convert(MUGPU_FP64, c->r64, MUGPU_FP32, c->r32, n, -s));
-    MUGPU_TRY(convert(MUGPU_FP64, c->r64, MUGPU_FP32, c->p32, n, s));
-    MUGPU_TRY(dot_device(MUGPU_FP32, c->r32, c->r32, n, c->partials, c->scal, FIN_STORE_RDR));
+    // r32 = -r64 s, p32 = -r32, x32 = 0, r32.r32: one pass over r64
+    MUGPU_TRY(mixed_start(res64, s, c->r32, c->p32, c->x32, n, c->partials, c->scal,
+                          FIN_STORE_RDR));
     MUGPU_TRY(sync_scalars(c->scal));
     const double rdr0 = g_host_scal[SC_RDR];
     const double tol2i = inner_tol * inner_tol * rdr0;
@@ -1347,9 +1347,9 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
       if (ran < (int)marks.size()) timing_void(marks[ran]);
     }
     its += g_host_ctl[CTL_ITER];
-    // the last iteration's x update (alpha and p of the last iteration run)
-    if (fuse) MUGPU_TRY(axpy_alpha(MUGPU_FP32, c->x32, c->p32, c->scal, n));
-    MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n));
+    // x += |r64| d, with the last iteration's x update (alpha and p of the
+    // last iteration run) folded in when it is pending
+    MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n, fuse ? c->p32 : nullptr, c->scal));
   }
 
   if (iterations) *iterations = its;

@@ -134,12 +134,56 @@ __global__ __launch_bounds__(kRT) void k_convert(const S* __restrict__ s, D* __r
     d[i] = (D)((double)s[i] * scale);
 }
 
+// start of an inner FP32 solve on A d = r64 / |r64| (s = 1/|r64|):
+// r32 = -s r64, p32 = -r32, x32 = 0 and the partials of r32.r32, reading r64
+// once (the partials in k_dot's order, so r.r is the same as k_dot's)
+__global__ __launch_bounds__(kRT) void k_mixed_start(const double* __restrict__ r64, double s,
+                                                     float* __restrict__ r32, float* __restrict__ p32,
+                                                     float* __restrict__ x32, int64_t n,
+                                                     double* partials) {
+  __shared__ double red[kRT];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const float r = (float)(r64[i] * -s);
+    r32[i] = r;
+    p32[i] = -r;
+    x32[i] = 0.0f;
+    acc += (double)r * (double)r;
+  }
+  const double t = block_reduce<kRT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// true residual: r = b - Ax (Ax in r on entry) and the partials of r.r
+__global__ __launch_bounds__(kRT) void k_residual(const double* __restrict__ b,
+                                                  double* __restrict__ r, int64_t n,
+                                                  double* partials) {
+  __shared__ double red[kRT];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kRT;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    const double v = b[i] - r[i];
+    r[i] = v;
+    acc += v * v;
+  }
+  const double t = block_reduce<kRT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// x += scale d; with p: d + alpha p first (alpha = scal[SC_ALPHA], in FP32
+// as the CG x update computes it), the x update left by the inner loop
 __global__ __launch_bounds__(kRT) void k_add_scaled(double* __restrict__ x,
                                                     const float* __restrict__ d, double scale,
-                                                    int64_t n) {
+                                                    int64_t n, const float* __restrict__ p,
+                                                    const double* __restrict__ scal) {
   const int64_t stride = (int64_t)gridDim.x * kRT;
-  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride)
-    x[i] += scale * (double)d[i];
+  const float al = p ? (float)scal[SC_ALPHA] : 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
+    float di = d[i];
+    if (p) di += al * p[i];
+    x[i] += scale * (double)di;
+  }
 }
 
 struct Uniform {
@@ -387,10 +431,29 @@ int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, d
   return MUGPU_OK;
 }
 
-int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n) {
+int mixed_start(const double* r64, double s, float* r32, float* p32, float* x32, int64_t n,
+                double* d_partials, double* d_scal, int mode) {
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_mixed_start, dim3(g), dim3(kRT), 0, stream(), r64, s, r32, p32, x32, n,
+                     d_partials);
+  MUGPU_HIP(hipGetLastError());
+  return finalize(d_partials, g, d_scal, mode);
+}
+
+int residual_dot(const double* b, double* r, int64_t n, double* d_partials, double* d_scal,
+                 int mode) {
+  const unsigned g = grid_for(n);
+  hipLaunchKernelGGL(k_residual, dim3(g), dim3(kRT), 0, stream(), b, r, n, d_partials);
+  MUGPU_HIP(hipGetLastError());
+  return finalize(d_partials, g, d_scal, mode);
+}
+
+int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n, const void* p32,
+                       const double* d_scal) {
+  if (p32 && !d_scal) return MUGPU_EARG;
   const unsigned g = grid_for(n);
   hipLaunchKernelGGL(k_add_scaled, dim3(g), dim3(kRT), 0, stream(), (double*)x64,
-                     (const float*)d32, scale, n);
+                     (const float*)d32, scale, n, (const float*)p32, d_scal);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }

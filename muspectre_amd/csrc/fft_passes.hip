@@ -493,8 +493,12 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
       set_error("last axis of %d points exceeds the supported line length", n);
       return MUGPU_EARG;
     }
+    // rows per workgroup: as many as fit LDS (up to 64) while the grid still
+    // has kMinBlocks workgroups; small grids get one row per workgroup
+    constexpr int64_t kMinBlocks = 2048;
     int rb = 1;
-    while (rb < rows && rb < 64 && fits(((rb + 1) * nc + 1) / 2, n, kNT) &&
+    while (rb < rows && rb < 64 && rows / (rb + 1) >= kMinBlocks &&
+           fits(((rb + 1) * nc + 1) / 2, n, kNT) &&
            fits((int64_t)(rb + 1) * (n / 2 + 1) * nc, 1, kNT * 64))
       ++rb;
     p.zrb = rb;
@@ -530,7 +534,9 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
       set_error("first axis of %d points exceeds the fused projection pass", nx);
       return MUGPU_EARG;
     }
+    // two k-points per workgroup where they fit and the grid stays large
     int Wk = 2;
+    if (p.kpts / nx < 2 * 2048) Wk = 1;
     while (Wk > 1 && !fits((int64_t)Wk * nc, nx, NT)) --Wk;
     p.xNT = NT;
     p.xWk = Wk;

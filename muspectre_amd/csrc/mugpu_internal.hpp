@@ -230,7 +230,16 @@ int scal_copy(double* d_scal, int from, int to);
 int cg_init_vectors(int prec, const void* Ax, const void* b, void* r, void* p, int64_t n);
 int axpby(int prec, double a, const void* x, double b, void* y, int64_t n);
 int convert(int from_prec, const void* src, int to_prec, void* dst, int64_t n, double scale);
-int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n);
+// x64 += scale d32; with p32: scale (d32 + scal[SC_ALPHA] p32)
+int add_scaled_convert(void* x64, const void* d32, double scale, int64_t n,
+                       const void* p32 = nullptr, const double* d_scal = nullptr);
+// inner-solve start of the mixed CG: r32 = -s r64, p32 = -r32, x32 = 0,
+// r32.r32 finalized into scal (mode as for finalize)
+int mixed_start(const double* r64, double s, float* r32, float* p32, float* x32, int64_t n,
+                double* d_partials, double* d_scal, int mode);
+// r = b - r (r holds A x on entry), r.r finalized into scal
+int residual_dot(const double* b, double* r, int64_t n, double* d_partials, double* d_scal,
+                 int mode);
 int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add);
 constexpr int kRedBlocks = 4096;  // capacity of the partial-sum buffers
 constexpr int kRedGrid = 1024;    // default workgroup cap of the grid-stride kernels
