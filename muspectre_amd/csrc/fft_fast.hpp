@@ -24,6 +24,8 @@
 // butterfly j < n/R, k = j mod NS:
 //   v_t = x[j + t n/R] W_{NS R}^{t k},  v = DFT_R(v),  y[(j/NS) NS R + k + t NS] = v_t
 #pragma once
+#include <type_traits>
+
 #include "mugpu_internal.hpp"
 #include "op_stages.hpp"
 

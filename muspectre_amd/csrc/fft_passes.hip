@@ -17,6 +17,7 @@
 // (N0, .., N_{d-1}/2+1) x nc complex.
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 
 #include "mugpu_internal.hpp"
 #include "op_stages.hpp"
@@ -87,34 +88,52 @@ __global__ __launch_bounds__(NT) void k_last_r2c(ZArgs<T> a) {
   } else {
     const int tot = rb * n;
     const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
-    for (int e = threadIdx.x; e < tot; e += NT) {
-      const int r = e / n;
-      const int m = e - r * n;
-      const int64_t row = row0 + r;
-      T dp[NC];
-      if (row < a.rows) {
-        const int64_t vox = row * n + m;
-        T p[NC];
-        if (IN == IN_TAN_PUPDATE) {
+    // one loop per tangent kind (the kind a compile-time constant in each,
+    // so the other kinds hold no registers inside the loop), and every load
+    // of a voxel ahead of its first store of p (p is updated in place)
+    auto voxels = [&](auto kind_c) {
+      constexpr int KIND = decltype(kind_c)::value;
+      for (int e = threadIdx.x; e < tot; e += NT) {
+        const int r = e / n;
+        const int m = e - r * n;
+        const int64_t row = row0 + r;
+        T dp[NC];
+        if (row < a.rows) {
+          const int64_t vox = row * n + m;
+          T p[NC];
+          if (IN == IN_TAN_PUPDATE) {
+            T rv[NC];
 #pragma unroll
-          for (int c = 0; c < NC; ++c) {
-            p[c] = -a.rvec[vox * NC + c] + beta * a.in[vox * NC + c];
-            a.pout[vox * NC + c] = p[c];
+            for (int c = 0; c < NC; ++c) {
+              rv[c] = a.rvec[vox * NC + c];
+              p[c] = a.in[vox * NC + c];
+            }
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+              p[c] = -rv[c] + beta * p[c];
+              a.pout[vox * NC + c] = p[c];
+            }
+          } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) p[c] = a.in[vox * NC + c];
           }
+          tangent_product<T, DIM>(KIND, a.tan, vox, p, dp);
         } else {
 #pragma unroll
-          for (int c = 0; c < NC; ++c) p[c] = a.in[vox * NC + c];
+          for (int c = 0; c < NC; ++c) dp[c] = T(0);
         }
-        tangent_product<T, DIM>(a.tan_kind, a.tan, vox, p, dp);
-      } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) dp[c] = T(0);
+        for (int c = 0; c < NC; ++c) {
+          const int L = r * NC + c;
+          sr[2 * ((L >> 1) * ls + m) + (L & 1)] = dp[c];
+        }
       }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const int L = r * NC + c;
-        sr[2 * ((L >> 1) * ls + m) + (L & 1)] = dp[c];
-      }
+    };
+    switch (a.tan_kind) {
+      case TAN_SYM: voxels(std::integral_constant<int, TAN_SYM>{}); break;
+      case TAN_HEP: voxels(std::integral_constant<int, TAN_HEP>{}); break;
+      case TAN_ISO: voxels(std::integral_constant<int, TAN_ISO>{}); break;
+      default: voxels(std::integral_constant<int, TAN_FULL>{}); break;
     }
   }
   if ((rb * NC) & 1) {
