@@ -247,9 +247,7 @@ def kernel_bytes(wl, kclass, prec):
     # iteration rides on the next iteration's r2c (x += alpha p_old where p
     # is read anyway), so the r2c moves x twice more and the update kernel
     # reads neither x nor p
-    # (FP32 inner solves; FP64 with the isotropic (lambda, mu) tangent of
-    # small strain)
-    fused = wl.res[-1] in FAST and (prec == "fp32" or wl.form == "small")
+    fused = wl.res[-1] in FAST
     if kclass == 0:
         # r and p read, p written (p = -r + beta p), [x read and written],
         # tangent read, half spectrum written

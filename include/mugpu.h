@@ -245,9 +245,9 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
  *   run (one whose r.r is NaN counts and stops the loop), *new_rdr the r.r
  *   of the last.  The host reads back once per call instead of once per
  *   iteration; the arithmetic is that of mugpu_cg_step.  Where the last
- *   axis has compiled passes and the tangent is isotropic, the x update of
- *   an iteration is applied by the next iteration's first pass (which reads
- *   p anyway), so the x update of the last iteration of a call is left
+ *   axis has compiled passes (lengths 16, 32, 48, 64, 96, 128, 256, 512),
+ *   the x update of an iteration is applied by the next iteration's first
+ *   pass (which reads p anyway), so the x update of the last iteration of a call is left
  *   pending (its alpha kept by the cell): the next call with update_p = 1
  *   on the same x and p applies it, and mugpu_cg_finish applies it after
  *   the last call; mugpu_cg_step, mugpu_cg_start and a call on other

@@ -1,6 +1,7 @@
 """Autotuner of the compile-time-length FFT passes.
 
-Every pass of the operator (last-axis r2c with the p update and K:p,
+Every pass of the operator (last-axis r2c with the p update, the previous
+iteration's x update and K:p,
 middle-axis c2c, first-axis c2c with the projection, last-axis c2r with the
 p.Ap partials, first-axis c2c of the engine transforms) exists in two
 variants for the power-of-two lengths: 16 complex values per thread, or 8

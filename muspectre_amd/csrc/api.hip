@@ -520,11 +520,13 @@ extern "C" int mugpu_fft_tune_pass(int dim, const int* res, int precision, int p
   auto run = [&]() -> int {
     switch (pass) {
       case FV_R2C: {
+        // the CG iteration's form: p update, the previous x update, K:p
         OpSpec op;
         op.in_mode = IN_TAN_PUPDATE;
         op.in = a;
         op.rvec = b;
         op.pout = a;
+        op.xvec = c;
         op.scal = scal;
         op.tan_kind = tk;
         op.tangent = tan;
@@ -1216,12 +1218,9 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
   int rc = MUGPU_OK;
   std::vector<int64_t> marks;
   // the x update of iteration i rides on the r2c of iteration i + 1 (x +=
-  // alpha p_old on the way in) where the last axis has compiled passes and
-  // that r2c stages p through LDS (isotropic tangent; with a 45/81-entry
-  // FP64 tangent it loads per voxel, and x would double its strided
-  // traffic); the last one is left pending for the next call or
-  // mugpu_cg_finish
-  const bool fuse = c->plan.fastax[c->dim - 1] && c->tan_kind == TAN_ISO;
+  // alpha p_old on the way in) where the last axis has compiled passes; the
+  // last one is left pending for the next call or mugpu_cg_finish
+  const bool fuse = c->plan.fastax[c->dim - 1];
   const bool pend_here = c->x_pending && c->x_pend == d_x && c->p_pend == d_p && update_p;
   if (c->x_pending && !pend_here) MUGPU_TRY(flush_x(c));
   // the first r2c applies the pending update with scal[SC_ALPHA]

@@ -376,7 +376,7 @@ struct LArgs {
   int64_t rows;
   const c_t<T>* tw;
   const int* active;  // FftPlan::active (single-cell kernels only)
-  T* x;               // OpSpec::xvec (IN_TAN_PUPDATE; nullptr: no x update)
+  T* x;               // OpSpec::xvec (IN_TAN_PUPDATE_X kernels only)
 };
 
 // ISO: the tangent is known to be isotropic (TAN_ISO), which the compiler
@@ -403,10 +403,11 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
     // tangent takes the per-voxel form below: there the workgroups are few
     // per CU, and the tangent loads of a voxel are better issued together
     // with its p and r loads than after a barrier.)
-    const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
+    constexpr bool PU = IN == IN_TAN_PUPDATE || IN == IN_TAN_PUPDATE_X;
     // the previous iteration's x update, x += alpha p_old, when fused here
-    const bool xupd = IN == IN_TAN_PUPDATE && a.x != nullptr;
-    const T al = xupd ? (T)a.scal[SC_ALPHA] : T(0);
+    constexpr bool XU = IN == IN_TAN_PUPDATE_X;
+    const T beta = PU ? (T)a.scal[SC_BETA] : T(0);
+    const T al = XU ? (T)a.scal[SC_ALPHA] : T(0);
     {
       constexpr int TOT = RB * N * NC, kBatch = 8;
       const int64_t base = row0 * N * NC, lim = a.rows * N * NC;
@@ -417,8 +418,8 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
           const int e = e0 + i * NT + threadIdx.x;
           const int64_t g = base + e < lim ? base + e : lim - 1;
           pv[i] = a.in[g];
-          if (IN == IN_TAN_PUPDATE) rv[i] = a.rvec[g];
-          if (IN == IN_TAN_PUPDATE && xupd) xv[i] = a.x[g];
+          if (PU) rv[i] = a.rvec[g];
+          if (XU) xv[i] = a.x[g];
         }
 #pragma unroll
         for (int i = 0; i < kBatch; ++i) {
@@ -427,8 +428,8 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
             const bool valid = base + e < lim;
             T x = T(0);
             if (valid) {
-              if (IN == IN_TAN_PUPDATE) {
-                if (xupd) a.x[base + e] = xv[i] + al * pv[i];
+              if (PU) {
+                if (XU) a.x[base + e] = xv[i] + al * pv[i];
                 x = -rv[i] + beta * pv[i];
                 a.pout[base + e] = x;
               } else {
@@ -460,8 +461,10 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
   } else {
     // one voxel per thread and step: p, r and tangent loads issued together
     // (FP64, tangent kind chosen at run time)
-    // (no fused x update here: the launcher refuses OpSpec::xvec for it)
-    const T beta = (IN == IN_TAN_PUPDATE) ? (T)a.scal[SC_BETA] : T(0);
+    constexpr bool PU = IN == IN_TAN_PUPDATE || IN == IN_TAN_PUPDATE_X;
+    constexpr bool XU = IN == IN_TAN_PUPDATE_X;
+    const T beta = PU ? (T)a.scal[SC_BETA] : T(0);
+    const T al = XU ? (T)a.scal[SC_ALPHA] : T(0);
     // one loop per tangent kind, the kind a compile-time constant in each:
     // a run-time switch inside the loop keeps the other kinds' values live
     // and leaves too few registers for the tangent loads, which then go out
@@ -475,16 +478,18 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
         if (row < a.rows) {
           const int64_t vox = row * N + m;
           T pv[NC];
-          if (IN == IN_TAN_PUPDATE) {
+          if (PU) {
             // all loads before the first store (p is updated in place)
-            T rv[NC];
+            T rv[NC], xv[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
               rv[c] = a.rvec[vox * NC + c];
               pv[c] = a.in[vox * NC + c];
+              if (XU) xv[c] = a.x[vox * NC + c];
             }
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
+              if (XU) a.x[vox * NC + c] = xv[c] + al * pv[c];
               pv[c] = -rv[c] + beta * pv[c];
               a.pout[vox * NC + c] = pv[c];
             }

@@ -25,16 +25,21 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   constexpr size_t lds = lds_lines<T, N, DIM, H>();
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   const bool iso = op.tan_kind == TAN_ISO;
-  // the fused x update exists only where p is staged through LDS
-  if (op.xvec && (op.in_mode != IN_TAN_PUPDATE || (sizeof(T) == 8 && !iso))) return MUGPU_EARG;
+  // the fused x update goes with the p update only
+  const bool xu = op.xvec != nullptr;
+  if (xu && op.in_mode != IN_TAN_PUPDATE) return MUGPU_EARG;
   if (op.in_mode == IN_PLAIN)
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_PLAIN, false, H>), grid, dim3(NT), lds, stream(), a);
   else if (op.in_mode == IN_TAN && iso)
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, true, H>), grid, dim3(NT), lds, stream(), a);
   else if (op.in_mode == IN_TAN)
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN, false, H>), grid, dim3(NT), lds, stream(), a);
+  else if (iso && xu)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE_X, true, H>), grid, dim3(NT), lds, stream(), a);
   else if (iso)
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, true, H>), grid, dim3(NT), lds, stream(), a);
+  else if (xu)
+    hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE_X, false, H>), grid, dim3(NT), lds, stream(), a);
   else
     hipLaunchKernelGGL((k_r2c_fast<T, N, E, DIM, IN_TAN_PUPDATE, false, H>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());

@@ -133,7 +133,9 @@ struct ProjParams {
 };
 
 // input stage of the last-axis r2c pass
-enum InMode { IN_PLAIN = 0, IN_TAN = 1, IN_TAN_PUPDATE = 2 };
+// IN_TAN_PUPDATE_X: kernel form of IN_TAN_PUPDATE that also applies x +=
+// alpha p_old (chosen by the launcher when OpSpec::xvec is set)
+enum InMode { IN_PLAIN = 0, IN_TAN = 1, IN_TAN_PUPDATE = 2, IN_TAN_PUPDATE_X = 3 };
 // output stage of the last-axis c2r pass
 enum OutMode { OUT_PLAIN = 0, OUT_DOT = 1 };
 

@@ -215,7 +215,8 @@ def _gpu_cg_chunked(gc, rhs, tol, maxiter, chunk):
     return x.download((gc.N, gc.nc)), r.download((gc.N, gc.nc)), it, rd, calls
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[6], CASES[9]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[5], CASES[6], CASES[7], CASES[9]],
+                         ids=lambda c: c[0])
 @pytest.mark.parametrize("chunk", [1, 3, 8])
 def test_cg_steps_equal_step_loop(capi, case, chunk):
     """Iterations queued on the device with the loop test there give the
@@ -249,7 +250,7 @@ def test_cg_steps_equal_step_loop(capi, case, chunk):
     assert conv1 is False and it1 == 7
 
 
-@pytest.mark.parametrize("case", [CASES[6], CASES[9], CASES[1]], ids=lambda c: c[0])
+@pytest.mark.parametrize("case", [CASES[6], CASES[7], CASES[9], CASES[1]], ids=lambda c: c[0])
 def test_cg_steps_pending_update_then_new_solve(capi, case):
     """A solve stopped after some chunks without mugpu_cg_finish: the next
     solve's mugpu_cg_start applies its pending x update with its own alpha
@@ -267,6 +268,8 @@ def test_cg_steps_pending_update_then_new_solve(capi, case):
     gc.add_pixels(mo, idx[~mask.ravel()])
     capi.check(gc.lib.mugpu_cell_initialise(gc.h))
     F = np.zeros((gc.N, gc.nc))
+    if form == 0:
+        F += np.eye(len(res)).ravel(order="F")
     F[:, 1] += 0.01
     gc.set_F(F)
     capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
