@@ -251,7 +251,9 @@ int mugpu_scale_copy(double a, const void *d_x, void *d_y, int64_t n);
  *   pending (its alpha kept by the cell): the next call with update_p = 1
  *   on the same x and p applies it, and mugpu_cg_finish applies it after
  *   the last call; mugpu_cg_step, mugpu_cg_start and a call on other
- *   vectors apply it first.  x and p must stay allocated until then.
+ *   vectors apply it first.  mugpu_free of that p applies it before the
+ *   block is released; mugpu_free of that x, or destroying the cell, drops
+ *   it.
  * mugpu_cg_finish: applies a pending x update (no-op otherwise).
  * The operator runs in FP64 (cells created with MUGPU_FP32 solve with
  * mugpu_cg_solve_mixed). */

@@ -131,6 +131,37 @@ struct mugpu_cell_s {
   void *x_pend = nullptr, *p_pend = nullptr;
 };
 
+// Cells with a pending x update.  mugpu_free consults them: freeing the p of
+// a pending update applies it first (p is still valid in stream order),
+// freeing its x drops it, so a later call never writes to a block that the
+// field cache has handed to someone else.
+namespace {
+std::mutex g_pend_mu;
+std::vector<mugpu_cell> g_pending_cells;
+
+void pending_set(mugpu_cell c, void* x, void* p) {
+  std::lock_guard<std::mutex> lk(g_pend_mu);
+  c->x_pending = true;
+  c->x_pend = x;
+  c->p_pend = p;
+  for (mugpu_cell q : g_pending_cells)
+    if (q == c) return;
+  g_pending_cells.push_back(c);
+}
+
+void pending_clear(mugpu_cell c) {
+  std::lock_guard<std::mutex> lk(g_pend_mu);
+  c->x_pending = false;
+  for (size_t i = 0; i < g_pending_cells.size(); ++i)
+    if (g_pending_cells[i] == c) {
+      g_pending_cells.erase(g_pending_cells.begin() + (std::ptrdiff_t)i);
+      break;
+    }
+}
+}  // namespace
+
+int flush_x(mugpu_cell c);
+
 namespace {
 
 int ensure_init() {
@@ -372,6 +403,24 @@ extern "C" int mugpu_alloc(size_t bytes, void** d_ptr) {
 
 extern "C" int mugpu_free(void* d_ptr) {
   if (!d_ptr) return MUGPU_OK;
+  {
+    // pending x updates that involve this block (see pending_set)
+    std::vector<mugpu_cell> apply;
+    {
+      std::lock_guard<std::mutex> lk(g_pend_mu);
+      for (size_t i = 0; i < g_pending_cells.size();) {
+        mugpu_cell c = g_pending_cells[i];
+        if (c->x_pend == d_ptr) {
+          c->x_pending = false;
+          g_pending_cells.erase(g_pending_cells.begin() + (std::ptrdiff_t)i);
+          continue;
+        }
+        if (c->p_pend == d_ptr) apply.push_back(c);
+        ++i;
+      }
+    }
+    for (mugpu_cell c : apply) MUGPU_TRY(flush_x(c));
+  }
   std::lock_guard<std::mutex> lk(g_alloc_mu);
   auto it = g_blocks.find(d_ptr);
   if (it == g_blocks.end()) {
@@ -742,6 +791,7 @@ extern "C" int mugpu_cell_create(int dim, const int* res, const double* lengths,
 
 extern "C" int mugpu_cell_destroy(mugpu_cell c) {
   if (!c) return MUGPU_OK;
+  pending_clear(c);  // a pending x update is dropped with the cell
   (void)hipStreamSynchronize(g_stream);
   for (auto& m : c->mats) {
     free_dev(m.d_pix);
@@ -1136,8 +1186,6 @@ extern "C" int mugpu_scale_copy(double a, const void* d_x, void* d_y, int64_t n)
 }
 
 // ============================== SolverCG ======================================
-int flush_x(mugpu_cell c);
-
 extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* d_r, void* d_p,
                               void* d_Ap, double* rdr, double* rhs_norm2) {
   if (!c) return MUGPU_EARG;
@@ -1163,7 +1211,7 @@ extern "C" int mugpu_cg_start(mugpu_cell c, const void* d_rhs, void* d_x, void* 
 // the x update left by mugpu_cg_steps: x += alpha p (alpha of its last iteration)
 int flush_x(mugpu_cell c) {
   if (!c->x_pending) return MUGPU_OK;
-  c->x_pending = false;
+  pending_clear(c);
   return axpy_alpha(MUGPU_FP64, c->x_pend, c->p_pend, c->scal, c->N * c->nc, SC_XALPHA);
 }
 
@@ -1225,7 +1273,7 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
   if (c->x_pending && !pend_here) MUGPU_TRY(flush_x(c));
   // the first r2c applies the pending update with scal[SC_ALPHA]
   if (pend_here) MUGPU_TRY(scal_copy(c->scal, SC_XALPHA, SC_ALPHA));
-  c->x_pending = false;
+  pending_clear(c);
   c->plan.active = c->ctl;
   for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) {
     marks.push_back(timing_mark());
@@ -1243,9 +1291,7 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
   if (fuse) {
     // alpha of the last iteration run (launches after the stop leave it)
     MUGPU_TRY(scal_copy(c->scal, SC_ALPHA, SC_XALPHA));
-    c->x_pending = true;
-    c->x_pend = d_x;
-    c->p_pend = d_p;
+    pending_set(c, d_x, d_p);
   }
   MUGPU_TRY(sync_ctl(c->ctl, c->scal));
   const double r = g_host_scal[SC_NEWRDR];

@@ -298,6 +298,56 @@ def test_cg_steps_pending_update_then_new_solve(capi, case):
     np.testing.assert_array_equal(x_b, ref_b)
 
 
+def test_cg_steps_pending_update_and_free(capi):
+    """Freeing the p of a pending x update (mugpu_free) applies it first;
+    freeing its x drops it, so a block the field cache hands out again is
+    never written by it."""
+    name, res, L, form, f_in, f_out, Ein, Eout, even = CASES[6]
+    gc = GpuCell(capi, res, L, form, allow_even=even)
+    mask = _sphere_mask(res, min(res) / 3)
+    mi = gc.add_material(f_in, Ein, .3)
+    mo = gc.add_material(f_out, Eout, .3)
+    idx = np.arange(mask.size)
+    gc.add_pixels(mi, idx[mask.ravel()])
+    gc.add_pixels(mo, idx[~mask.ravel()])
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    F = np.zeros((gc.N, gc.nc))
+    F[:, 1] += 0.01
+    gc.set_F(F)
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    lib = gc.lib
+    n = gc.N * gc.nc
+    rhs = np.random.default_rng(3).standard_normal((gc.N, gc.nc))
+    ref, it, _ = _gpu_cg(gc, rhs, 1e-30, 5)
+    assert it == 5
+
+    def start():
+        b = DeviceBuffer.from_array(rhs)
+        x, r, p, Ap = (DeviceBuffer(8 * n) for _ in range(4))
+        rdr, bn2 = ctypes.c_double(), ctypes.c_double()
+        capi.check(lib.mugpu_cg_start(gc.h, b.ptr, x.ptr, r.ptr, p.ptr, Ap.ptr, ctypes.byref(rdr),
+                                      ctypes.byref(bn2)))
+        nr, done = ctypes.c_double(), ctypes.c_int()
+        capi.check(lib.mugpu_cg_steps(gc.h, 0, x.ptr, r.ptr, p.ptr, Ap.ptr, 0.0, 5,
+                                      ctypes.byref(done), ctypes.byref(nr)))
+        assert done.value == 5
+        return b, x, r, p, Ap
+
+    b, x, r, p, Ap = start()
+    p.free()  # applies the pending update
+    np.testing.assert_array_equal(x.download((gc.N, gc.nc)), ref)
+    b, x, r, p, Ap = start()
+    x.free()  # drops it
+    y = DeviceBuffer(8 * n)  # may reuse x's block
+    marker = np.arange(n, dtype=np.float64)
+    y.upload(marker)
+    capi.check(lib.mugpu_cg_finish(gc.h))
+    b2, x2, r2, p2, Ap2 = start()
+    capi.check(lib.mugpu_cg_finish(gc.h))
+    np.testing.assert_array_equal(y.download(n), marker)
+    np.testing.assert_array_equal(x2.download((gc.N, gc.nc)), ref)
+
+
 def _aniso_voigt(dim, seed, major_sym=True):
     vs = 3 if dim == 2 else 6
     rng = np.random.default_rng(seed)
