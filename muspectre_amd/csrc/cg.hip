@@ -299,7 +299,9 @@ int batch_finalize(const BatchSlot* d_slots, int nslots, int64_t count, int mode
 
 int batch_cg_update(const BatchSlot* d_slots, int nslots, int64_t n) {
   if (nslots <= 0) return MUGPU_OK;
-  const unsigned g = grid_for(n);
+  // the grid of the unbatched FP32 update (cg_update without x), so that
+  // r.r is summed in the same order
+  const unsigned g = grid_for(n, kRedBlocks);
   timing_begin(MUGPU_K_CG_UPDATE);
   hipLaunchKernelGGL(k_cg_update_batch, dim3(g, 1, (unsigned)nslots), dim3(kRT), 0, stream(),
                      d_slots, n);
@@ -313,7 +315,7 @@ int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots) {
   MUGPU_TRY(fast_batch_operator(p, d_slots, nslots));
   MUGPU_TRY(batch_finalize(d_slots, nslots, p.nzb[1], FIN_ALPHA));
   MUGPU_TRY(batch_cg_update(d_slots, nslots, n));
-  return batch_finalize(d_slots, nslots, grid_for(n), FIN_BETA);
+  return batch_finalize(d_slots, nslots, grid_for(n, kRedBlocks), FIN_BETA);
 }
 
 int reduction_blocks(int64_t n) { return (int)grid_for(n); }
