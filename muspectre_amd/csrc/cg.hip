@@ -318,7 +318,6 @@ int batch_cg_iteration(FftPlan& p, const BatchSlot* d_slots, int nslots) {
   return batch_finalize(d_slots, nslots, grid_for(n, kRedBlocks), FIN_BETA);
 }
 
-int reduction_blocks(int64_t n) { return (int)grid_for(n); }
 
 int copy_device(void* dst, const void* src, size_t bytes) {
   const int64_t n = (int64_t)(bytes / 16);

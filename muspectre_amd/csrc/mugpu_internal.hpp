@@ -245,8 +245,6 @@ int residual_dot(const double* b, double* r, int64_t n, double* d_partials, doub
 int fill_uniform(double* F, const double* h_val, int nc, int64_t npix, int add);
 constexpr int kRedBlocks = 4096;  // capacity of the partial-sum buffers
 constexpr int kRedGrid = 1024;    // default workgroup cap of the grid-stride kernels
-// workgroups (= partial sums) of the reductions over n values
-int reduction_blocks(int64_t n);
 
 // ---- materials (materials.hip) ----------------------------------------------
 struct MaterialDev {
