@@ -1,9 +1,9 @@
 """Build step of the package: compiles libmugpu.so (HIP, gfx950), the
 pybind11 module _muSpectre and the CPU oracle used by the tests.
 
-Run as ``python muspectre_amd/build.py`` or through __graft_entry__.build().
-Every call recompiles (make decides per object file what is out of date;
-``force=True`` rebuilds everything).
+Run as ``python muspectre_amd/build.py`` (make decides per object file what
+is out of date; ``--force`` rebuilds everything) or through
+__graft_entry__.build(), which always rebuilds everything.
 """
 import glob
 import os
@@ -46,14 +46,14 @@ def build_module():
     return out
 
 
-def build_oracle():
-    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+def build_oracle(force=False):
+    _run(["make", "-C", os.path.join(ROOT, "oracle")] + (["-B"] if force else []))
 
 
 def build_all(force=False):
     build_hip(force=force)
     build_module()
-    build_oracle()
+    build_oracle(force=force)
 
 
 if __name__ == "__main__":
