@@ -279,8 +279,12 @@ __device__ void pass_generic(c_t<T>* s, int nl, int ls, int n, int Ns, int R,
       const int e = k + q * Ns;
       const c_t<T>* src = s + line * ls + j;
       c_t<T> a = src[0];
+      // ex = t e mod span, kept by adding e (e < span) instead of a 64-bit
+      // multiply and modulo per term
+      int ex = 0;
       for (int t = 1; t < R; ++t) {
-        const int ex = (int)(((long long)t * e) % span);
+        ex += e;
+        if (ex >= span) ex -= span;
         a = cadd(a, cmul(src[t * nb], twid(tw, ex * step, sign)));
       }
       acc[i] = a;
