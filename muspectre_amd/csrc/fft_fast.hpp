@@ -304,19 +304,20 @@ __device__ __forceinline__ void axis0_body(const XArgs<T>& a, unsigned char* sme
   Passes<T, N, E, -1, 0, 1>::run(v, p, a.tw, at);
   store_last<T, N, E>(v, p, [&](int m, c_t<T> x) { at(m) = x; });
   __syncthreads();
-  // projection per wave vector (k0, q0 + kp)
-  for (int e = threadIdx.x; e < N * KP; e += NT) {
-    const int k0 = e / KP, kp = e - (e / KP) * KP;
-    const int64_t q = q0 + kp;
-    if (q >= a.plane) continue;
-    int c[3];
+  // projection per wave vector (k0, q0 + kp); NT is a multiple of KP, so a
+  // thread keeps one kp and its (k1, k2) for every k0 it visits (a plane
+  // holds fewer than 2^31 k-points)
+  const int kp = threadIdx.x % KP;
+  const int64_t q = q0 + kp;
+  int c[3] = {0, 0, 0};
+  if (DIM == 3) {
+    c[1] = (int)q / a.hlast;
+    c[2] = (int)q - c[1] * a.hlast;
+  } else {
+    c[1] = (int)q;
+  }
+  for (int k0 = threadIdx.x / KP; k0 < N && q < a.plane; k0 += NT / KP) {
     c[0] = k0;
-    if (DIM == 3) {
-      c[1] = (int)(q / a.hlast);
-      c[2] = (int)(q - (int64_t)c[1] * a.hlast);
-    } else {
-      c[1] = (int)q;
-    }
     T xi[DIM];
     c_t<T>* f = s + k0 * LS + kp * NC;
     if (!unit_wave_vector<T, DIM>(c, a.res, a.L, a.allow_even, xi)) {

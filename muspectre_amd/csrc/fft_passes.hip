@@ -350,8 +350,9 @@ __global__ __launch_bounds__(NT) void k_axis0_project(PArgs<T> a) {
     int c[3];
     c[0] = k0;
     if (DIM == 3) {
-      c[1] = (int)(q / a.hlast);
-      c[2] = (int)(q - (int64_t)c[1] * a.hlast);
+      // a plane holds fewer than 2^31 k-points
+      c[1] = (int)q / a.hlast;
+      c[2] = (int)q - c[1] * a.hlast;
     } else {
       c[1] = (int)q;
     }
