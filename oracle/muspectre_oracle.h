@@ -19,9 +19,13 @@
  *   - both transforms unnormalised; 1/N folded into the projection
  *     (src/fft/fft_engine_base.cc:49, src/fft/projection_default.cc:56-60)
  *
- * Parity pinning of each function is listed in DESIGN.md; functions whose
- * result no golden vector, closed form or invariant fixes are marked
- * "parity unpinned" there and below.
+ * Each function is checked by tests/test_oracle_*.py against the reference's
+ * golden vectors (tests/golden/reference_vectors.json), closed forms,
+ * invariants, finite differences or brute force.  Parity unpinned (nothing
+ * of that kind fixes them): logm/expm at exactly repeated eigenvalues against
+ * Eigen's computeDirect rounding (the mathematical function is matched to
+ * 1e-13); the reference's own Newton/CG iteration counts at the benchmark
+ * configurations; HEP1 history over more steps than the two golden states.
  */
 #ifndef MUSPECTRE_ORACLE_H
 #define MUSPECTRE_ORACLE_H
