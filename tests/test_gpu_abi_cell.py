@@ -99,6 +99,8 @@ CASES = [
     ("3d_finite_le1_mixed", [16, 9, 32], [1., 1., 1.], 0, 1, 1, 210e9, 70e9, 1),
     ("3d_small_le1_radix3", [48, 16, 96], [1., 1., 1.], 1, 1, 1, 100e9, 10e9, 1),
     ("2d_small_le1_radix3", [96, 48], [1., 1.], 1, 1, 1, 100e9, 10e9, 1),
+    # runtime-length last axis under 4096+ rows: several rows per workgroup
+    ("3d_finite_le1_rows", [64, 72, 15], [1., 1., 1.], 0, 1, 1, 210e9, 70e9, 1),
 ]
 
 

@@ -21,7 +21,10 @@ SIZES = [[6, 4], [4, 6], [5, 5], [3, 3], [51, 51], [17, 4], [3, 5, 7], [7, 7, 7]
          [125, 5, 3], [2, 2, 2], [1, 10], [64, 48, 40], [256, 128], [16, 512],
          [128, 32, 64], [16, 256, 32], [32, 9, 16], [512, 3, 16], [96, 16, 64],
          [3, 512], [64, 64], [7, 128, 5], [48, 48, 48], [96, 32, 48], [48, 96],
-         [96, 96, 96], [5, 48, 96]]
+         [96, 96, 96], [5, 48, 96],
+         # runtime-length last axis with more rows than 2 x 2048: several rows
+         # per workgroup
+         [96, 64, 17], [128, 64, 9]]
 
 
 def _fft_plan(capi, res, prec):
