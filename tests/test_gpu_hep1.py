@@ -28,9 +28,9 @@ def _grf_mask(res, ell, seed):
     return f > np.median(f)
 
 
-def _cells(µ, oracle, res, consistent):
+def _cells(µ, oracle, res, consistent, even=False):
     mask = _grf_mask(res, 2.0, 1)
-    cell = µ.Cell(res, [1., 1., 1.], µ.Formulation.finite_strain)
+    cell = µ.Cell(res, [1., 1., 1.], µ.Formulation.finite_strain, allow_even=even)
     mk = µ.material.MaterialHyperElastoPlastic1_3d.make
     soft = mk(cell, "soft", E_SOFT, NU, TY_SOFT, H, consistent_tangent=consistent)
     hard = mk(cell, "hard", E_HARD, NU, TY_HARD, H, consistent_tangent=consistent)
@@ -38,7 +38,7 @@ def _cells(µ, oracle, res, consistent):
     hard.add_pixels_by_index(idx[mask.ravel()])
     soft.add_pixels_by_index(idx[~mask.ravel()])
     fam = oracle.HEP1_CT if consistent else oracle.HEP1
-    oc = oracle.Cell(res, [1., 1., 1.], oracle.FINITE_STRAIN)
+    oc = oracle.Cell(res, [1., 1., 1.], oracle.FINITE_STRAIN, allow_even=even)
     a = oc.add_material(fam, E_SOFT, NU, TY_SOFT, H)
     b = oc.add_material(fam, E_HARD, NU, TY_HARD, H)
     for i in range(mask.size):
@@ -101,3 +101,39 @@ def test_hep1_consistent_c3_first_step_parity(µ, oracle, res):
     assert np.linalg.norm(r.stress.ravel() - ro["stress"].ravel()) <= 1e-9 * np.linalg.norm(ro["stress"])
     eps = cell.get_globalised_internal_real_array("eps_p").ravel()
     assert np.abs(eps - oc.get_internal("eps_p", 0).ravel()).max() <= 1e-10
+
+
+def test_hep1_consistent_c3_first_step_parity_compiled_lengths(µ, oracle):
+    """The C3 path as benchmarked: axes with compiled passes (even policy),
+    the 45-entry consistent tangent and the x update carried by the next
+    iteration's first pass; same Newton and CG counts and fields as the
+    oracle."""
+    res = [16, 16, 32]
+    cell, oc = _cells(µ, oracle, res, True, even=True)
+    load = np.diag([0., 0., 0.005])
+    solver = µ.solvers.SolverCG(cell, 1e-8, 2000, 0)
+    r = µ.solvers.newton_cg(cell, [load], solver, 1e-5, 1e-10, 0)[0]
+    ro = oc.newton_cg([load], 1e-8, 2000, 1e-5, 1e-10)[0]
+    assert r.nb_it == ro["nb_it"]
+    assert abs(int(r.nb_fev) - int(ro["nb_fev"])) <= 1
+    assert np.linalg.norm(r.grad.ravel() - ro["grad"].ravel()) <= 1e-10 * np.linalg.norm(ro["grad"])
+    assert np.linalg.norm(r.stress.ravel() - ro["stress"].ravel()) <= 1e-9 * np.linalg.norm(ro["stress"])
+
+
+def test_hep1_as_coded_tangent_compiled_lengths(µ, oracle):
+    """The coded 27-entry HEP1 tangent through the compiled-length operator:
+    the directional stiffness G:(K:dF) matches the oracle's."""
+    res = [16, 32, 16]
+    cell, oc = _cells(µ, oracle, res, False, even=True)
+    cell.initialise()
+    oc.initialise()
+    N = int(np.prod(res))
+    rng = np.random.default_rng(9)
+    F = np.eye(3).ravel(order="F") + 0.003 * rng.standard_normal((N, 9))
+    cell.evaluate_stress_tangent(F.T)
+    oc.F[:] = F
+    oc.evaluate_stress_tangent()
+    dF = rng.standard_normal((N, 9))
+    g = np.asarray(cell.directional_stiffness(dF.T)).T.reshape(N, 9)
+    o = oc.directional_stiffness(dF)
+    assert np.abs(g - o).max() <= 1e-10 * np.abs(o).max()
