@@ -68,13 +68,16 @@ def test_c1_disc_fp32_refined(µ, oracle):
     assert solver.get_refinements() >= 2
 
 
+@pytest.mark.parametrize("res", [[9, 11, 7], [16, 32, 16]], ids=lambda r: "x".join(map(str, r)))
 @pytest.mark.parametrize("form", ["small", "finite"])
-def test_3d_two_phase_fp32_refined(µ, oracle, form):
-    res = [9, 11, 7]
+def test_3d_two_phase_fp32_refined(µ, oracle, form, res):
+    """Runtime-length and compiled-length axes (the latter with the x update
+    on the next first pass and the even policy)."""
+    even = any(n % 2 == 0 for n in res)
     g = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
     mask = ((g[0] - 4) ** 2 + (g[1] - 5) ** 2 + (g[2] - 3) ** 2) < 9
     F_ = µ.Formulation.small_strain if form == "small" else µ.Formulation.finite_strain
-    cell = µ.Cell(res, [1., 1.2, .8], F_, precision="fp32")
+    cell = µ.Cell(res, [1., 1.2, .8], F_, precision="fp32", allow_even=even)
     h = µ.material.MaterialLinearElastic1_3d.make(cell, "hard", 100e9, .3)
     s = µ.material.MaterialLinearElastic1_3d.make(cell, "soft", 10e9, .3)
     idx = np.arange(mask.size)
@@ -83,7 +86,8 @@ def test_3d_two_phase_fp32_refined(µ, oracle, form):
     steps = [np.array([[.01, .002, 0], [.002, 0, 0], [0, 0, -.005]]) * k for k in (1, 2)]
     solver = µ.solvers.SolverCG(cell, 1e-8, 400, 0)
     rs = µ.solvers.newton_cg(cell, steps, solver, 1e-5, 1e-10, 0)
-    oc = oracle.Cell(res, [1., 1.2, .8], oracle.SMALL_STRAIN if form == "small" else oracle.FINITE_STRAIN)
+    oc = oracle.Cell(res, [1., 1.2, .8], oracle.SMALL_STRAIN if form == "small" else oracle.FINITE_STRAIN,
+                     allow_even=even)
     a = oc.add_material(oracle.LE1, 100e9, .3)
     b = oc.add_material(oracle.LE1, 10e9, .3)
     for i in range(mask.size):
@@ -112,19 +116,23 @@ def test_laminate_fp32_refined(µ):
         assert abs(F[i, 0] - ref) < 1e-7 and np.abs(F[i, 1:]).max() < 1e-7
 
 
-def test_hep1_consistent_fp32_refined(µ, oracle):
-    """C3 first load step at 9^3 with the consistent tangent, FP32 operator."""
+@pytest.mark.parametrize("res", [[9, 9, 9], [16, 16, 32]], ids=lambda r: "x".join(map(str, r)))
+def test_hep1_consistent_fp32_refined(µ, oracle, res):
+    """C3 first load step with the consistent tangent, FP32 operator; 9^3
+    (runtime-length axes) and 16x16x32 (compiled-length axes: the
+    benchmark's FP32 path)."""
     from test_gpu_hep1 import _grf_mask, E_SOFT, E_HARD, NU, TY_SOFT, TY_HARD, H
-    res = [9, 9, 9]
+    even = any(n % 2 == 0 for n in res)
     mask = _grf_mask(res, 2.0, 1)
-    cell = µ.Cell(res, [1., 1., 1.], µ.Formulation.finite_strain, precision="fp32")
+    cell = µ.Cell(res, [1., 1., 1.], µ.Formulation.finite_strain, precision="fp32",
+                  allow_even=even)
     mk = µ.material.MaterialHyperElastoPlastic1_3d.make
     soft = mk(cell, "soft", E_SOFT, NU, TY_SOFT, H, consistent_tangent=True)
     hard = mk(cell, "hard", E_HARD, NU, TY_HARD, H, consistent_tangent=True)
     idx = np.arange(mask.size)
     hard.add_pixels_by_index(idx[mask.ravel()])
     soft.add_pixels_by_index(idx[~mask.ravel()])
-    oc = oracle.Cell(res, [1., 1., 1.], oracle.FINITE_STRAIN)
+    oc = oracle.Cell(res, [1., 1., 1.], oracle.FINITE_STRAIN, allow_even=even)
     a = oc.add_material(oracle.HEP1_CT, E_SOFT, NU, TY_SOFT, H)
     b = oc.add_material(oracle.HEP1_CT, E_HARD, NU, TY_HARD, H)
     for i in range(mask.size):
