@@ -396,14 +396,11 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
       const int m = rem / NC, c = rem - m * NC;
       sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = base + e < lim ? a.in[base + e] : T(0);
     }
-  } else if constexpr (sizeof(T) == 4 || ISO) {
+  } else {
     // p (updated on the way in: p = -r + beta p) is staged in LDS by an
     // element-wise sweep, consecutive lanes on consecutive values, so that
     // the AoS fields are read and written in full cache lines; clamped
-    // unconditional loads, kBatch per thread in flight.  (FP64 with a large
-    // tangent takes the per-voxel form below: there the workgroups are few
-    // per CU, and the tangent loads of a voxel are better issued together
-    // with its p and r loads than after a barrier.)
+    // unconditional loads, kBatch per thread in flight.
     constexpr bool PU = IN == IN_TAN_PUPDATE || IN == IN_TAN_PUPDATE_X;
     // the previous iteration's x update, x += alpha p_old, when fused here
     constexpr bool XU = IN == IN_TAN_PUPDATE_X;
@@ -446,72 +443,34 @@ __device__ __forceinline__ void r2c_body(const LArgs<T>& a, unsigned char* smem,
     }
     __syncthreads();
     // K:p per voxel, in place: each voxel's components are read and written
-    // by one thread
-    for (int e = threadIdx.x; e < RB * N; e += NT) {
-      const int r = e / N, m = e - (e / N) * N;
-      const int64_t row = row0 + r;
-      if (row < a.rows) {
-        T pv[NC], dp[NC];
-#pragma unroll
-        for (int c = 0; c < NC; ++c) pv[c] = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
-        tangent_product<T, DIM>(ISO ? TAN_ISO : a.tan_kind, a.tan, row * N + m, pv, dp);
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
-      }
-    }
-  } else {
-    // one voxel per thread and step: p, r and tangent loads issued together
-    // (FP64, tangent kind chosen at run time)
-    constexpr bool PU = IN == IN_TAN_PUPDATE || IN == IN_TAN_PUPDATE_X;
-    constexpr bool XU = IN == IN_TAN_PUPDATE_X;
-    const T beta = PU ? (T)a.scal[SC_BETA] : T(0);
-    const T al = XU ? (T)a.scal[SC_ALPHA] : T(0);
-    // one loop per tangent kind, the kind a compile-time constant in each:
-    // a run-time switch inside the loop keeps the other kinds' values live
-    // and leaves too few registers for the tangent loads, which then go out
-    // one at a time
-    auto voxels = [&](auto kind_c) {
+    // by one thread.  One loop per tangent kind, the kind a compile-time
+    // constant in each: a run-time switch inside the loop keeps the other
+    // kinds' values live and leaves too few registers for the tangent loads,
+    // which then go out one at a time.
+    auto kp_loop = [&](auto kind_c) {
       constexpr int KIND = decltype(kind_c)::value;
       for (int e = threadIdx.x; e < RB * N; e += NT) {
         const int r = e / N, m = e - (e / N) * N;
         const int64_t row = row0 + r;
-        T dp[NC];
         if (row < a.rows) {
-          const int64_t vox = row * N + m;
-          T pv[NC];
-          if (PU) {
-            // all loads before the first store (p is updated in place)
-            T rv[NC], xv[NC];
+          T pv[NC], dp[NC];
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              rv[c] = a.rvec[vox * NC + c];
-              pv[c] = a.in[vox * NC + c];
-              if (XU) xv[c] = a.x[vox * NC + c];
-            }
+          for (int c = 0; c < NC; ++c) pv[c] = sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)];
+          tangent_product<T, DIM>(KIND, a.tan, row * N + m, pv, dp);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) {
-              if (XU) a.x[vox * NC + c] = xv[c] + al * pv[c];
-              pv[c] = -rv[c] + beta * pv[c];
-              a.pout[vox * NC + c] = pv[c];
-            }
-          } else {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) pv[c] = a.in[vox * NC + c];
-          }
-          tangent_product<T, DIM>(KIND, a.tan, vox, pv, dp);
-        } else {
-#pragma unroll
-          for (int c = 0; c < NC; ++c) dp[c] = T(0);
+          for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
         }
-#pragma unroll
-        for (int c = 0; c < NC; ++c) sr[2 * (m * LS + (r >> 1) * NC + c) + (r & 1)] = dp[c];
       }
     };
-    switch (a.tan_kind) {
-      case TAN_SYM: voxels(std::integral_constant<int, TAN_SYM>{}); break;
-      case TAN_HEP: voxels(std::integral_constant<int, TAN_HEP>{}); break;
-      case TAN_ISO: voxels(std::integral_constant<int, TAN_ISO>{}); break;
-      default: voxels(std::integral_constant<int, TAN_FULL>{}); break;
+    if constexpr (ISO) {
+      kp_loop(std::integral_constant<int, TAN_ISO>{});
+    } else {
+      switch (a.tan_kind) {
+        case TAN_SYM: kp_loop(std::integral_constant<int, TAN_SYM>{}); break;
+        case TAN_HEP: kp_loop(std::integral_constant<int, TAN_HEP>{}); break;
+        case TAN_ISO: kp_loop(std::integral_constant<int, TAN_ISO>{}); break;
+        default: kp_loop(std::integral_constant<int, TAN_FULL>{}); break;
+      }
     }
   }
   __syncthreads();
