@@ -599,6 +599,28 @@ static int ensure_cg(mo_cell *c) {
   return (c->cg_r && c->cg_p && c->cg_Ap) ? MO_OK : MO_ALLOC_ERROR;
 }
 
+/* stopping-test records of mo_cg_solve (diagnostics, see the header) */
+#define MO_CG_RECORDS_MAX 4096
+static double g_cg_rec[MO_CG_RECORDS_MAX][4];
+static int g_cg_nrec = 0;
+
+int mo_cg_records(double *out, int cap) {
+  const int n = g_cg_nrec < cap ? g_cg_nrec : cap;
+  memcpy(out, g_cg_rec, sizeof(double) * 4 * (size_t)(n > 0 ? n : 0));
+  return g_cg_nrec;
+}
+
+void mo_cg_records_clear(void) { g_cg_nrec = 0; }
+
+static void cg_record(int iters, double rdr, double rdr_prev, double tol2) {
+  if (g_cg_nrec >= MO_CG_RECORDS_MAX) return;
+  g_cg_rec[g_cg_nrec][0] = iters;
+  g_cg_rec[g_cg_nrec][1] = rdr;
+  g_cg_rec[g_cg_nrec][2] = rdr_prev;
+  g_cg_rec[g_cg_nrec][3] = tol2;
+  ++g_cg_nrec;
+}
+
 /* src/solver/solver_cg.cc:54-110 */
 int mo_cg_solve(mo_cell *c, double tol, int maxiter, const double *rhs,
                 double *x, long *counter) {
@@ -615,6 +637,8 @@ int mo_cg_solve(mo_cell *c, double tol, int maxiter, const double *rhs,
   double rdr = dot(r, r, n);
   const double rhs_norm2 = dot(rhs, rhs, n);
   const double tol2 = tol * tol * rhs_norm2;
+  double rdr_prev = rdr;
+  int its = 0;
   for (int i = 0; i < maxiter && (rdr > tol2 || i == 0); ++i, ++(*counter)) {
     e = mo_cell_directional_stiffness(c, p, Ap);
     if (e) return e;
@@ -626,10 +650,13 @@ int mo_cg_solve(mo_cell *c, double tol, int maxiter, const double *rhs,
     }
     const double new_rdr = dot(r, r, n);
     const double beta = new_rdr / rdr;
+    rdr_prev = rdr;
     rdr = new_rdr;
+    ++its;
 #pragma omp parallel for schedule(static) num_threads(mo_get_threads())
     for (long q = 0; q < n; ++q) p[q] = -r[q] + beta * p[q];
   }
+  cg_record(its, rdr, rdr_prev, tol2);
   if (rdr < tol2) return MO_OK;
   mo_set_error("Conjugate gradient has not converged. After %ld steps, the solver  FAILED with  |r|/|b| = %g, cg_tol = %g",
                *counter, sqrt(rdr / rhs_norm2), tol);

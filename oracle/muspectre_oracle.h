@@ -21,11 +21,15 @@
  *
  * Each function is checked by tests/test_oracle_*.py against the reference's
  * golden vectors (tests/golden/reference_vectors.json), closed forms,
- * invariants, finite differences or brute force.  Parity unpinned (nothing
- * of that kind fixes them): logm/expm at exactly repeated eigenvalues against
- * Eigen's computeDirect rounding (the mathematical function is matched to
- * 1e-13); the reference's own Newton/CG iteration counts at the benchmark
- * configurations; HEP1 history over more steps than the two golden states.
+ * invariants, finite differences or brute force.  The coded Kirchhoff->PK1
+ * tangent (mo_pk1_from_kirchhoff), for which no reference vector exists, is
+ * fixed uniquely by two identities on random inputs: K(i,m,j,n) does not
+ * depend on j, and F K over its first index equals the sums of C over its
+ * last index.  Parity unpinned (nothing of that kind fixes them):
+ * logm/expm at exactly repeated eigenvalues against Eigen's computeDirect
+ * rounding (the mathematical function is matched to 1e-13); the reference's
+ * own Newton/CG iteration counts at the benchmark configurations; HEP1
+ * history over more steps than the two golden states.
  */
 #ifndef MUSPECTRE_ORACLE_H
 #define MUSPECTRE_ORACLE_H
@@ -166,6 +170,16 @@ int mo_cell_get_internal(mo_cell *c, const char *name, int which,
  * solver's cumulative counter and is updated. */
 int mo_cg_solve(mo_cell *c, double tol, int maxiter, const double *rhs,
                 double *x, long *counter);
+
+/* Diagnostics of the stopping test (no reference counterpart): every
+ * mo_cg_solve appends one record {iterations, r.r after the last iteration,
+ * r.r after the one before (r.r at the start for a single iteration),
+ * tol2}, so that a test can see how close r.r came to tol2 where the loop
+ * stopped.  mo_cg_records copies up to cap records (4 doubles each) and
+ * returns how many are held; mo_cg_records_clear empties the list (at most
+ * 4096 records are kept). */
+int mo_cg_records(double *out, int cap);
+void mo_cg_records_clear(void);
 
 typedef struct {
   int success;

@@ -96,6 +96,8 @@ def lib():
         L.mo_pk1_from_pk2.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.mo_pk1_from_kirchhoff.argtypes = [ctypes.c_int, dp, dp, dp, dp, dp]
         L.mo_set_threads.argtypes = [ctypes.c_int]
+        L.mo_cg_records.argtypes = [dp, ctypes.c_int]
+        L.mo_cg_records.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -115,6 +117,18 @@ def _check(code):
 
 def set_threads(n):
     lib().mo_set_threads(int(n))
+
+
+def cg_records(clear=True):
+    """Stopping-test records of the oracle's CG solves since the last clear:
+    an array of rows (iterations, r.r at the stop, r.r one iteration
+    earlier, tol2)."""
+    n = lib().mo_cg_records(_dp(np.zeros(4)), 0)
+    out = np.zeros((max(n, 1), 4))
+    lib().mo_cg_records(_dp(out), n)
+    if clear:
+        lib().mo_cg_records_clear()
+    return out[:n]
 
 
 def hermitian_sizes(res):

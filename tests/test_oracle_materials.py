@@ -202,6 +202,37 @@ def test_hep1_coded_tangent_is_j_independent(oracle):
     assert np.abs(K4 - K4[:, :, :1, :]).max() == 0.0
 
 
+@pytest.mark.parametrize("d", [2, 3])
+@pytest.mark.parametrize("seed", range(3))
+def test_kirchhoff_to_pk1_determined_by_identities(oracle, d, seed):
+    """The coded Kirchhoff -> PK1 conversion (materials_toolbox.hh:356-410)
+    on random inputs with no symmetry: P = tau F^-T, and the tangent
+    K(i,m,j,n) = sum_r Finv(i,r) sum_s C(r,m,n,s), which is fixed uniquely by
+    two identities that do not restate it: K does not depend on j, and
+    contracting its first index with F from the left gives the sums of C
+    over its last index, sum_i F(k,i) K(i,m,j,n) = sum_s C(k,m,n,s) (F is
+    invertible, so no other K satisfies this).  A transposed inverse,
+    Finv(r,i), or a swapped pair of C's indices fails it."""
+    import ctypes
+    rng = np.random.default_rng(100 + seed)
+    F = np.eye(d) + 0.3 * rng.standard_normal((d, d))
+    tau = rng.standard_normal((d, d))
+    C = rng.standard_normal(d ** 4)
+    Ff, tf = np.asfortranarray(F), np.asfortranarray(tau)
+    P = np.zeros((d, d), order="F")
+    K = np.zeros(d ** 4)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    oracle.lib().mo_pk1_from_kirchhoff(d, dp(Ff), dp(tf), dp(C), dp(P), dp(K))
+    K4 = K.reshape(d, d, d, d, order="F")  # (i, m, j, n)
+    C4 = C.reshape(d, d, d, d, order="F")  # (r, m, n, s)
+    assert np.abs(P @ F.T - tau).max() <= 1e-13 * np.abs(tau).max()
+    assert np.abs(K4 - K4[:, :, :1, :]).max() == 0.0
+    S = C4.sum(axis=3)  # (k, m, n)
+    FK = np.einsum("ki,imjn->kmjn", F, K4)
+    for j in range(d):
+        assert np.abs(FK[:, :, j, :] - S).max() <= 1e-12 * np.abs(S).max()
+
+
 def _generic_cell(oracle, dim, form, Cv, F):
     """A 3^dim cell of one MaterialLinearElasticGeneric under a uniform F;
     returns P and K of pixel 0."""
