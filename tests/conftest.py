@@ -18,6 +18,9 @@ def pytest_configure(config):
 def oracle():
     from oracle import oracle as O
     O.lib()
+    # the oracle's OpenMP loops on at most 16 host threads (the CPU share of
+    # one GPU box); only the summation order of its dot products depends on it
+    O.set_threads(min(16, os.cpu_count() or 1))
     return O
 
 

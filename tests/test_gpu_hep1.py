@@ -88,15 +88,27 @@ def test_hep1_as_coded_c3_step_raises_like_oracle(µ, oracle):
         µ.solvers.newton_cg(cell, load, solver, 1e-5, 1e-10, 0)
 
 
+def _margins(oracle):
+    """The oracle's CG stopping margins since the last call, for the message
+    of a failed count comparison: r.r / tol2 at the stop and one iteration
+    before, per solve."""
+    return "; ".join("{:d} its: {:.6f} / {:.6f}".format(int(q[0]), q[1] / q[3], q[2] / q[3])
+                     for q in oracle.cg_records())
+
+
 @pytest.mark.parametrize("res", [[9, 9, 9], [15, 13, 11]], ids=lambda r: "x".join(map(str, r)))
 def test_hep1_consistent_c3_first_step_parity(µ, oracle, res):
+    """Newton and CG counts equal to the oracle's.  The 15x13x11 case stops
+    one of its CG solves at r.r / tol2 = 0.9986: the count is equal as long as
+    the GPU's residual history stays within 0.14 % of the oracle's."""
     cell, oc = _cells(µ, oracle, res, True)
     load = np.diag([0., 0., 0.005])
     solver = µ.solvers.SolverCG(cell, 1e-8, 2000, 0)
     r = µ.solvers.newton_cg(cell, [load], solver, 1e-5, 1e-10, 0)[0]
+    oracle.cg_records()
     ro = oc.newton_cg([load], 1e-8, 2000, 1e-5, 1e-10)[0]
     assert r.nb_it == ro["nb_it"]
-    assert abs(int(r.nb_fev) - int(ro["nb_fev"])) <= 1
+    assert int(r.nb_fev) == int(ro["nb_fev"]), _margins(oracle)
     assert np.linalg.norm(r.grad.ravel() - ro["grad"].ravel()) <= 1e-10 * np.linalg.norm(ro["grad"])
     assert np.linalg.norm(r.stress.ravel() - ro["stress"].ravel()) <= 1e-9 * np.linalg.norm(ro["stress"])
     eps = cell.get_globalised_internal_real_array("eps_p").ravel()
@@ -113,9 +125,10 @@ def test_hep1_consistent_c3_first_step_parity_compiled_lengths(µ, oracle):
     load = np.diag([0., 0., 0.005])
     solver = µ.solvers.SolverCG(cell, 1e-8, 2000, 0)
     r = µ.solvers.newton_cg(cell, [load], solver, 1e-5, 1e-10, 0)[0]
+    oracle.cg_records()
     ro = oc.newton_cg([load], 1e-8, 2000, 1e-5, 1e-10)[0]
     assert r.nb_it == ro["nb_it"]
-    assert abs(int(r.nb_fev) - int(ro["nb_fev"])) <= 1
+    assert int(r.nb_fev) == int(ro["nb_fev"]), _margins(oracle)
     assert np.linalg.norm(r.grad.ravel() - ro["grad"].ravel()) <= 1e-10 * np.linalg.norm(ro["grad"])
     assert np.linalg.norm(r.stress.ravel() - ro["stress"].ravel()) <= 1e-9 * np.linalg.norm(ro["stress"])
 

@@ -151,3 +151,29 @@ def test_admission_keeps_budget(serving):
         q = _request([16, 16, 16], 2)
         srv.submit([15, 16, 16], np.ones(15 * 256), np.full(15 * 256, .3), q["strain"])
     srv.close()
+
+
+@pytest.mark.parametrize("res", [[32, 32, 32], [64, 64]], ids=lambda r: "x".join(map(str, r)))
+def test_served_request_vs_oracle(serving, oracle, res):
+    """A served request against the oracle's FP64 newton_cg on the same
+    per-voxel LE4 cell and strain (the serving loop runs FP32 CG with FP64
+    refinement): Newton count equal, homogenised stress to 1e-6, strain
+    field to 1e-5, both solves to cg_tol 1e-8."""
+    q = _request(res, 11)
+    out, _ = _serve(serving, [q], cg_tol=1e-8, cg_maxiter=4000)
+    r = out[0]
+    d = len(res)
+    nc = d * d
+    oc = oracle.Cell(res, [1.0] * d, oracle.SMALL_STRAIN, allow_even=True)
+    m = oc.add_material(oracle.LE4)
+    for i in range(q["E"].size):
+        oc.add_pixel(m, i, [q["E"][i], q["nu"][i]])
+    o = oc.newton_cg([q["strain"]], 1e-8, 4000, 1e-5, 1e-10)[0]
+    ref = o["stress"].mean(0)
+    es = np.linalg.norm(r["mean_stress"].ravel() - ref) / np.linalg.norm(ref)
+    eg = np.linalg.norm(r["strain"].reshape(-1, nc) - o["grad"]) / np.linalg.norm(o["grad"])
+    print("served {}: stress {:.2e}, strain {:.2e}".format("x".join(map(str, res)), es, eg),
+          flush=True)
+    assert r["ok"] and r["newton_iterations"] == o["nb_it"]
+    assert es <= 1e-6
+    assert eg <= 1e-5
