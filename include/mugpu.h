@@ -215,6 +215,26 @@ int mugpu_cell_save_history(mugpu_cell cell);
 int mugpu_cell_get_internal(mugpu_cell cell, const char *name, double *h_out,
                             int *nb_components);
 
+/* MaterialMuSpectre's internal state fields (material_muSpectre_base.hh,
+ * StateField src/common/statefield.hh:206-326) of a HyperElastoPlastic1
+ * material: one sub-field ("<prefix>, sub_field index i", i = 0, 1) of
+ * "cumulated plastic flow εₚ" ("eps_p", 1 component), "Previous
+ * placement gradient Fᵗ" ("F_prev", nc) or "Previous left Cauchy-Green
+ * deformation bₑᵗ" ("be", nc), in the material's own pixel order
+ * (the order of mugpu_material_add_pixels), into h_out (nb_pixels of the
+ * material x nb_components; NULL to ask only for the shape).
+ * *current_sub_field returns the sub-field that is current: 1 after
+ * mugpu_cell_initialise (material_hyper_elasto_plastic1.cc:76-84), swapped by
+ * each mugpu_cell_save_history (statefield.hh:318-326). */
+int mugpu_material_get_state(mugpu_cell cell, int mat, const char *name,
+                             int sub_field, double *h_out, int *nb_components,
+                             int *current_sub_field);
+/* the operator of the cell's projection (CellBase::get_projection,
+ * src/cell/cell_base.hh:323, ProjectionBase::get_operator,
+ * projection_base.hh:144): Kpts x (nc*nc) for small strain, Kpts x dim for
+ * the fast finite-strain projection, as mugpu_projection_get_operator */
+int mugpu_cell_projection_operator(mugpu_cell cell, double *h_out);
+
 /* ---- vector kernels for the Newton driver (src/solver/solvers.cc:
  * 143-145,173-175,184-187; Communicator::sum is the identity on one GPU,
  * src/common/communicator.hh:122-124) ------------------------------------ */
@@ -265,6 +285,14 @@ int mugpu_cg_steps(mugpu_cell cell, int update_p, void *d_x, void *d_r,
                    void *d_p, void *d_Ap, double tol2, int max_steps,
                    int *steps, double *new_rdr);
 int mugpu_cg_finish(mugpu_cell cell);
+/* mugpu_cg_steps on cells of at most 2^20 values (nb_pixels x nb_components)
+ * replays each chunk from a captured hipGraph (kept on the cell per set of
+ * vectors, chunk length and form of the first iteration; the arithmetic is
+ * the same), because there the host's kernel launches, not the device, set
+ * the pace.  Not while kernels are timed one by one (mugpu_timing_start).
+ * enable = 0 turns this off for the process, 1 (the default) on.  No
+ * reference counterpart. */
+int mugpu_cg_graphs(int enable);
 
 /* ---- mixed-precision CG with FP64 refinement (no reference counterpart;
  * it replaces SolverCG::solve for cells created with MUGPU_FP32) --------
@@ -380,6 +408,13 @@ int mugpu_fft_tune_pass(int dim, const int *resolutions, int precision, int pass
 /* streaming device copy of `bytes` (multiple of 16), the bandwidth
  * reference for the roofline and for the HBM counter calibration */
 int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);
+/* achievable device bandwidth: GB/s (bytes read + written) of four copies
+ * of `bytes` between two fresh device blocks, each timed with HIP events over
+ * `reps` launches after one untimed launch: gbs[0] mugpu_copy's kernel, gbs[1]
+ * a streaming copy with four 16-byte loads in flight per lane, gbs[2] the
+ * same with non-temporal loads and stores, gbs[3] hipMemcpyAsync device to
+ * device */
+int mugpu_copy_bandwidth(size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

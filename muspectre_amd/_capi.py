@@ -78,6 +78,8 @@ SIGNATURES = [
     ("mugpu_cell_apply_projection", _int, [_vp, _vp]),
     ("mugpu_cell_save_history", _int, [_vp]),
     ("mugpu_cell_get_internal", _int, [_vp, ctypes.c_char_p, _dp, _ip]),
+    ("mugpu_material_get_state", _int, [_vp, _int, ctypes.c_char_p, _int, _dp, _ip, _ip]),
+    ("mugpu_cell_projection_operator", _int, [_vp, _dp]),
     ("mugpu_dot", _int, [_vp, _vp, _i64, _dp]),
     ("mugpu_axpby", _int, [_dbl, _vp, _dbl, _vp, _i64]),
     ("mugpu_scale_copy", _int, [_dbl, _vp, _vp, _i64]),
@@ -85,6 +87,7 @@ SIGNATURES = [
     ("mugpu_cg_step", _int, [_vp, _int, _vp, _vp, _vp, _vp, _dp]),
     ("mugpu_cg_steps", _int, [_vp, _int, _vp, _vp, _vp, _vp, _dbl, _int, _ip, _dp]),
     ("mugpu_cg_finish", _int, [_vp]),
+    ("mugpu_cg_graphs", _int, [_int]),
     ("mugpu_cg_solve_mixed", _int, [_vp, _vp, _vp, _dbl, _int, _dbl, _i64p, _ip, _dp]),
     ("mugpu_timing_start", _int, [_int]),
     ("mugpu_timing_stop", _int, [_i64p, _dp]),
@@ -100,6 +103,7 @@ SIGNATURES = [
     ("mugpu_server_release", _int, [_vp, _i64]),
     ("mugpu_server_stats", _int, [_vp, _i64p, _int]),
     ("mugpu_copy", _int, [_vp, _vp, ctypes.c_size_t]),
+    ("mugpu_copy_bandwidth", _int, [ctypes.c_size_t, _int, _dp]),
 ]
 K_LAST_R2C, K_MIDDLE, K_AXIS0, K_LAST_C2R, K_MATERIAL, K_CG_UPDATE = range(6)
 K_ALL = 6

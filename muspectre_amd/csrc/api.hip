@@ -46,6 +46,10 @@ void set_capturing(bool on) { g_capturing = on; }
 int max_lds_per_block() { return g_maxlds; }
 
 namespace {
+// graph replay of mugpu_cg_steps chunks on cells of at most this many values
+constexpr int64_t kCgGraphMaxValues = int64_t(1) << 20;
+constexpr size_t kCgGraphsPerCell = 16;
+bool g_cg_graphs = true;
 int g_timing_class = -1;
 std::vector<hipEvent_t> g_ev_start, g_ev_stop;
 int64_t g_timing_count = 0;
@@ -87,6 +91,7 @@ void timing_void(int64_t from) {
 }
 
 int copy_device(void* dst, const void* src, size_t bytes);
+int copy_device_wide(void* dst, const void* src, size_t bytes, bool nontemporal, int grid);
 
 int run_tangent_product(int dim, int prec, int kind, const void* tan, const void* in, void* out,
                         int64_t npix);
@@ -129,6 +134,14 @@ struct mugpu_cell_s {
   // (to x_pend with p_pend) by the next call or by mugpu_cg_finish
   bool x_pending = false;
   void *x_pend = nullptr, *p_pend = nullptr;
+  // captured chunks of mugpu_cg_steps (small cells), by vectors, chunk
+  // length and form of the first iteration
+  struct CgGraph {
+    void *x, *r, *p, *Ap;
+    int steps, first_update_p, first_x;
+    hipGraphExec_t exec;
+  };
+  std::vector<CgGraph> graphs;
 };
 
 // Cells with a pending x update.  mugpu_free consults them: freeing the p of
@@ -266,37 +279,58 @@ int cell_operator64(mugpu_cell c, const void* in, void* out) {
   return run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
 }
 
-// control words of the device-resident CG loops (CTL_*)
-int ensure_ctl(mugpu_cell c) {
-  if (!c->ctl && hipMalloc((void**)&c->ctl, sizeof(int) * CTL_COUNT) != hipSuccess) {
-    set_error("device allocation of the CG control words failed");
-    return MUGPU_EMEMORY;
-  }
-  return MUGPU_OK;
-}
+
+// work vectors of the mixed-precision solve: FP32 CG vectors, the FP64
+// residual and the control words of the device-resident inner loop
+struct MixedVecs {
+  float *x32 = nullptr, *r32 = nullptr, *p32 = nullptr, *Ap32 = nullptr;
+  double* r64 = nullptr;
+  int* ctl = nullptr;
+};
 
 // one iteration of the FP32 inner CG (solver_cg.cc:62-96): A p fused with
 // the p update (all but the first iteration), p.Ap, the x and r updates and
-// r.r, gated by the cell's control words.  fuse: the previous iteration's
-// x update rides on this iteration's r2c (x += alpha p_old on the way in)
-// instead of the CG update kernel, which then reads neither x nor p
-int inner_iteration(mugpu_cell c, bool first, bool fuse, int64_t n) {
+// r.r, gated by the control words.  fuse: the previous iteration's x update
+// rides on this iteration's r2c (x += alpha p_old on the way in) instead of
+// the CG update kernel, which then reads neither x nor p
+int inner_iteration(mugpu_cell c, const MixedVecs& v, bool first, bool fuse, int64_t n) {
   OpSpec op;
   op.in_mode = first ? IN_TAN : IN_TAN_PUPDATE;
-  op.xvec = fuse && !first ? c->x32 : nullptr;
-  op.in = c->p32;
-  op.rvec = c->r32;
-  op.pout = c->p32;
+  op.xvec = fuse && !first ? v.x32 : nullptr;
+  op.in = v.p32;
+  op.rvec = v.r32;
+  op.pout = v.p32;
   op.scal = c->scal;
   op.tan_kind = c->tan_kind;
   op.tangent = c->tan32;
   op.out_mode = OUT_DOT;
-  op.out = c->Ap32;
-  op.dotv = c->p32;
+  op.out = v.Ap32;
+  op.dotv = v.p32;
   MUGPU_TRY(run_operator(c->plan, MUGPU_FP32, op, c->pj, c->spectrum));
-  MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzb[1], c->scal, FIN_ALPHA, c->ctl));
-  return cg_update(MUGPU_FP32, fuse ? nullptr : c->x32, c->r32, c->p32, c->Ap32, c->scal, n,
-                   c->partials, c->ctl);
+  MUGPU_TRY(finalize((const double*)c->plan.partials, c->plan.nzb[1], c->scal, FIN_ALPHA, v.ctl));
+  return cg_update(MUGPU_FP32, fuse ? nullptr : v.x32, v.r32, v.p32, v.Ap32, c->scal, n,
+                   c->partials, v.ctl);
+}
+
+// the mixed vectors of a cell, allocated together: after a failed attempt
+// none is kept, so a retry allocates them all again
+int ensure_mixed(mugpu_cell c, int64_t n) {
+  if (c->x32 && c->r32 && c->p32 && c->Ap32 && c->r64) return MUGPU_OK;
+  auto drop = [&]() {
+    free_dev(c->x32);
+    free_dev(c->r32);
+    free_dev(c->p32);
+    free_dev(c->Ap32);
+    free_dev(c->r64);
+  };
+  drop();
+  int rc = dev_alloc(&c->x32, sizeof(float) * n, "the mixed-precision CG vectors");
+  if (!rc) rc = dev_alloc(&c->r32, sizeof(float) * n, "the mixed-precision CG vectors");
+  if (!rc) rc = dev_alloc(&c->p32, sizeof(float) * n, "the mixed-precision CG vectors");
+  if (!rc) rc = dev_alloc(&c->Ap32, sizeof(float) * n, "the mixed-precision CG vectors");
+  if (!rc) rc = dev_alloc(&c->r64, sizeof(double) * n, "the mixed-precision CG vectors");
+  if (rc) drop();
+  return rc;
 }
 
 }  // namespace
@@ -366,6 +400,30 @@ void release_cached_locked() {
   g_cached_bytes = 0;
 }
 }  // namespace
+
+int mugpu::dev_alloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    if (!g_cached.empty()) {
+      (void)hipStreamSynchronize(g_stream);
+      release_cached_locked();
+      e = hipMalloc(p, bytes ? bytes : 16);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+      }
+    }
+  }
+  if (e != hipSuccess) {
+    set_error("device allocation of %s (%zu bytes) failed: %s", what, bytes, hipGetErrorString(e));
+    return MUGPU_EMEMORY;
+  }
+  return MUGPU_OK;
+}
 
 extern "C" int mugpu_alloc(size_t bytes, void** d_ptr) {
   MUGPU_TRY(ensure_init());
@@ -473,6 +531,11 @@ extern "C" int mugpu_synchronize(void) {
 }
 
 extern "C" void* mugpu_stream(void) { return (void*)g_stream; }
+
+extern "C" int mugpu_cg_graphs(int enable) {
+  g_cg_graphs = enable != 0;
+  return MUGPU_OK;
+}
 
 extern "C" int mugpu_timing_start(int kernel_class) {
   MUGPU_TRY(ensure_init());
@@ -643,6 +706,50 @@ extern "C" int mugpu_copy(void* d_dst, const void* d_src, size_t bytes) {
   return copy_device(d_dst, d_src, bytes);
 }
 
+extern "C" int mugpu_copy_bandwidth(size_t bytes, int reps, double* gbs) {
+  MUGPU_TRY(ensure_init());
+  if (!gbs || reps < 1 || bytes < 16 || bytes % 16) {
+    set_error("mugpu_copy_bandwidth needs reps >= 1 and a multiple of 16 bytes");
+    return MUGPU_EARG;
+  }
+  int cus = 0;
+  MUGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, g_device));
+  void *a = nullptr, *b = nullptr;
+  MUGPU_TRY(dev_alloc(&a, bytes, "the copy source"));
+  int rc = dev_alloc(&b, bytes, "the copy destination");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MUGPU_EHIP;
+  if (!rc && hipMemsetAsync(a, 1, bytes, g_stream) != hipSuccess) rc = MUGPU_EHIP;
+  for (int v = 0; v < 4 && !rc; ++v) {
+    auto run = [&]() -> int {
+      switch (v) {
+        case 0: return copy_device(b, a, bytes);
+        case 1: return copy_device_wide(b, a, bytes, false, cus * 8);
+        case 2: return copy_device_wide(b, a, bytes, true, cus * 8);
+        default:
+          return hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, g_stream) == hipSuccess
+                     ? MUGPU_OK
+                     : MUGPU_EHIP;
+      }
+    };
+    rc = run();  // warm-up
+    if (!rc && hipEventRecord(e0, g_stream) != hipSuccess) rc = MUGPU_EHIP;
+    for (int i = 0; i < reps && !rc; ++i) rc = run();
+    if (!rc && (hipEventRecord(e1, g_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+      rc = MUGPU_EHIP;
+    float ms = 0.f;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MUGPU_EHIP;
+    if (!rc) gbs[v] = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  }
+  if (rc == MUGPU_EHIP) set_error("HIP error in the bandwidth measurement");
+  (void)hipStreamSynchronize(g_stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  free_dev(a);
+  free_dev(b);
+  return rc;
+}
+
 // ============================== FFT engine ====================================
 extern "C" int mugpu_fft_create(int dim, const int* res, int nb_components, int precision,
                                 mugpu_fft* out) {
@@ -704,10 +811,10 @@ extern "C" int mugpu_projection_create(mugpu_fft engine, int kind, const double*
   p->pj.norm = 1.0 / (double)engine->plan.N;
   const size_t bytes = (size_t)engine->plan.kpts * engine->plan.nc * 2 *
                        (engine->plan.prec == MUGPU_FP64 ? sizeof(double) : sizeof(float));
-  if (hipMalloc(&p->spectrum, bytes) != hipSuccess) {
+  const int rc = dev_alloc(&p->spectrum, bytes, "the projection work space");
+  if (rc) {
     delete p;
-    set_error("device allocation of the projection work space failed");
-    return MUGPU_EMEMORY;
+    return rc;
   }
   *out = p;
   return MUGPU_OK;
@@ -773,14 +880,18 @@ extern "C" int mugpu_cell_create(int dim, const int* res, const double* lengths,
   c->pj.norm = 1.0 / (double)c->N;
   const size_t fb = sizeof(double) * (size_t)c->N * c->nc;
   const size_t sb = sizeof(double) * 2 * (size_t)c->plan.kpts * c->nc;
-  bool ok = hipMalloc((void**)&c->F, fb) == hipSuccess && hipMalloc((void**)&c->P, fb) == hipSuccess &&
-            hipMalloc((void**)&c->tmp, fb) == hipSuccess && hipMalloc(&c->spectrum, sb) == hipSuccess &&
-            hipMalloc((void**)&c->scal, sizeof(double) * SC_COUNT) == hipSuccess &&
-            hipMalloc((void**)&c->partials, sizeof(double) * kRedBlocks) == hipSuccess;
-  if (!ok) {
+  rc = dev_alloc(&c->F, fb, "the strain field");
+  if (!rc) rc = dev_alloc(&c->P, fb, "the stress field");
+  if (!rc) rc = dev_alloc(&c->tmp, fb, "the cell's work field");
+  if (!rc) rc = dev_alloc(&c->spectrum, sb, "the cell's spectrum");
+  if (!rc) rc = dev_alloc(&c->scal, sizeof(double) * SC_COUNT, "the CG scalars");
+  if (!rc) rc = dev_alloc(&c->partials, sizeof(double) * kRedBlocks, "the partial sums");
+  if (!rc) rc = dev_alloc(&c->ctl, sizeof(int) * CTL_COUNT, "the CG control words");
+  if (rc) {
+    const std::string msg = g_err;
     mugpu_cell_destroy(c);
-    set_error("device allocation of the cell fields failed");
-    return MUGPU_EMEMORY;
+    set_error("%s", msg.c_str());
+    return rc;
   }
   (void)hipMemsetAsync(c->F, 0, fb, g_stream);
   (void)hipMemsetAsync(c->P, 0, fb, g_stream);
@@ -817,6 +928,7 @@ extern "C" int mugpu_cell_destroy(mugpu_cell c) {
   free_dev(c->Ap32);
   free_dev(c->r64);
   free_dev(c->ctl);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
   plan_free(c->plan);
   delete c;
   return MUGPU_OK;
@@ -955,29 +1067,36 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
                 : all_sym                                 ? TAN_SYM
                                                           : TAN_FULL;
   const size_t tn = (size_t)tangent_values(c->tan_kind, c->dim, c->N);
-  if (hipMalloc((void**)&c->tan64, tn * sizeof(double)) != hipSuccess) {
-    set_error("device allocation of the tangent failed");
-    return MUGPU_EMEMORY;
-  }
+  // an initialise that failed part way is retried from scratch: what the
+  // failed call allocated is released first
+  free_dev(c->tan64);
+  free_dev(c->tan32);
+  MUGPU_TRY(dev_alloc(&c->tan64, tn * sizeof(double), "the tangent"));
   MUGPU_HIP(hipMemsetAsync(c->tan64, 0, tn * sizeof(double), g_stream));
-  if (c->prec == MUGPU_FP32) {
-    if (hipMalloc((void**)&c->tan32, tn * sizeof(float)) != hipSuccess) {
-      set_error("device allocation of the FP32 tangent failed");
-      return MUGPU_EMEMORY;
+  if (c->prec == MUGPU_FP32) MUGPU_TRY(dev_alloc(&c->tan32, tn * sizeof(float), "the FP32 tangent"));
+  const int nc = c->nc;
+  for (auto& m : c->mats) {
+    free_dev(m.d_pix);
+    free_dev(m.d_params);
+    free_dev(m.d_C);
+    for (int b = 0; b < 2; ++b) {
+      free_dev(m.Fp[b]);
+      free_dev(m.be[b]);
+      free_dev(m.ep[b]);
     }
   }
-  const int nc = c->nc;
   for (auto& m : c->mats) {
     const int64_t n = (int64_t)m.h_pix.size();
     std::vector<int> pix32(m.h_pix.begin(), m.h_pix.end());
-    MUGPU_HIP(hipMalloc((void**)&m.d_pix, sizeof(int) * (size_t)(n > 0 ? n : 1)));
+    MUGPU_TRY(dev_alloc(&m.d_pix, sizeof(int) * (size_t)(n > 0 ? n : 1), "a material's pixel list"));
     if (n) MUGPU_HIP(hipMemcpy(m.d_pix, pix32.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     if (m.family == MUGPU_LEG) {
-      MUGPU_HIP(hipMalloc((void**)&m.d_C, sizeof(double) * m.h_C.size()));
+      MUGPU_TRY(dev_alloc(&m.d_C, sizeof(double) * m.h_C.size(), "a material's stiffness"));
       MUGPU_HIP(hipMemcpy(m.d_C, m.h_C.data(), sizeof(double) * m.h_C.size(), hipMemcpyHostToDevice));
     }
     if (m.ppn) {
-      MUGPU_HIP(hipMalloc((void**)&m.d_params, sizeof(double) * (size_t)(n > 0 ? n * m.ppn : 1)));
+      MUGPU_TRY(dev_alloc(&m.d_params, sizeof(double) * (size_t)(n > 0 ? n * m.ppn : 1),
+                          "a material's per-pixel parameters"));
       if (n) MUGPU_HIP(hipMemcpy(m.d_params, m.h_params.data(), sizeof(double) * n * m.ppn,
                                  hipMemcpyHostToDevice));
     }
@@ -986,9 +1105,9 @@ extern "C" int mugpu_cell_initialise(mugpu_cell c) {
       // then cycle; both buffers start at zero
       const size_t sz = (size_t)(n > 0 ? n : 1);
       for (int b = 0; b < 2; ++b) {
-        MUGPU_HIP(hipMalloc((void**)&m.Fp[b], sizeof(double) * sz * nc));
-        MUGPU_HIP(hipMalloc((void**)&m.be[b], sizeof(double) * sz * nc));
-        MUGPU_HIP(hipMalloc((void**)&m.ep[b], sizeof(double) * sz));
+        MUGPU_TRY(dev_alloc(&m.Fp[b], sizeof(double) * sz * nc, "the HEP1 state"));
+        MUGPU_TRY(dev_alloc(&m.be[b], sizeof(double) * sz * nc, "the HEP1 state"));
+        MUGPU_TRY(dev_alloc(&m.ep[b], sizeof(double) * sz, "the HEP1 state"));
         MUGPU_HIP(hipMemsetAsync(m.Fp[b], 0, sizeof(double) * sz * nc, g_stream));
         MUGPU_HIP(hipMemsetAsync(m.be[b], 0, sizeof(double) * sz * nc, g_stream));
         MUGPU_HIP(hipMemsetAsync(m.ep[b], 0, sizeof(double) * sz, g_stream));
@@ -1164,6 +1283,54 @@ extern "C" int mugpu_cell_get_internal(mugpu_cell c, const char* name, double* h
   return MUGPU_OK;
 }
 
+extern "C" int mugpu_material_get_state(mugpu_cell c, int mat, const char* name, int sub_field,
+                                        double* h_out, int* nb_components,
+                                        int* current_sub_field) {
+  if (!c || !name || mat < 0 || mat >= (int)c->mats.size()) return MUGPU_EARG;
+  const MaterialDev& m = c->mats[(size_t)mat];
+  if (m.family != MUGPU_HEP1 && m.family != MUGPU_HEP1_CT) {
+    set_error("material %d holds no state fields", mat);
+    return MUGPU_ERUNTIME;
+  }
+  if (!m.Fp[0]) {
+    set_error("the state fields exist once the cell is initialised");
+    return MUGPU_ERUNTIME;
+  }
+  if (sub_field < 0 || sub_field > 1) {
+    set_error("sub_field index %d: a state field with one old value has sub-fields 0 and 1",
+              sub_field);
+    return MUGPU_EARG;
+  }
+  // short names or the reference's prefixes (material_hyper_elasto_plastic1.cc:46-53)
+  const std::string nm(name);
+  const double* src = nullptr;
+  int k = 0;
+  if (nm == "eps_p" || nm == "cumulated plastic flow \xce\xb5\xe2\x82\x9a") {
+    src = m.ep[sub_field];
+    k = 1;
+  } else if (nm == "F_prev" || nm == "Previous placement gradient F\xe1\xb5\x97") {
+    src = m.Fp[sub_field];
+    k = c->nc;
+  } else if (nm == "be" || nm == "Previous left Cauchy-Green deformation b\xe2\x82\x91\xe1\xb5\x97") {
+    src = m.be[sub_field];
+    k = c->nc;
+  } else {
+    set_error("no state field '%s' in this material", name);
+    return MUGPU_ERUNTIME;
+  }
+  if (nb_components) *nb_components = k;
+  if (current_sub_field) *current_sub_field = m.cur;
+  if (h_out && !m.h_pix.empty())
+    MUGPU_TRY(mugpu_download(h_out, src, sizeof(double) * m.h_pix.size() * (size_t)k));
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_cell_projection_operator(mugpu_cell c, double* h_out) {
+  if (!c || !h_out) return MUGPU_EARG;
+  host_operator(c->pj.kind, c->dim, c->res, c->pj.L, c->pj.allow_even, h_out);
+  return MUGPU_OK;
+}
+
 // ============================== vector kernels ================================
 extern "C" int mugpu_dot(const void* d_a, const void* d_b, int64_t n, double* h_out) {
   MUGPU_TRY(ensure_init());
@@ -1249,7 +1416,6 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
                               double* new_rdr) {
   if (!c || max_steps < 1 || !steps || !new_rdr) return MUGPU_EARG;
   const int64_t n = c->N * c->nc;
-  MUGPU_TRY(ensure_ctl(c));
   MUGPU_TRY(set_ctl(c->ctl, max_steps, c->scal, tol2));
   OpSpec op;
   op.in = d_p;
@@ -1275,16 +1441,58 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
   if (pend_here) MUGPU_TRY(scal_copy(c->scal, SC_XALPHA, SC_ALPHA));
   pending_clear(c);
   c->plan.active = c->ctl;
-  for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) {
-    marks.push_back(timing_mark());
+  auto queue = [&](int i) -> int {
     op.in_mode = (i > 0 || update_p) ? IN_TAN_PUPDATE : IN_TAN;
     op.xvec = fuse && (i > 0 || pend_here) ? d_x : nullptr;
-    rc = run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
-    if (rc == MUGPU_OK)
-      rc = finalize((const double*)c->plan.partials, c->plan.nzb[0], c->scal, FIN_ALPHA, c->ctl);
-    if (rc == MUGPU_OK)
-      rc = cg_update(MUGPU_FP64, fuse ? nullptr : d_x, d_r, d_p, d_Ap, c->scal, n, c->partials,
-                     c->ctl);
+    int e = run_operator(c->plan, MUGPU_FP64, op, c->pj, c->spectrum);
+    if (e == MUGPU_OK)
+      e = finalize((const double*)c->plan.partials, c->plan.nzb[0], c->scal, FIN_ALPHA, c->ctl);
+    if (e == MUGPU_OK)
+      e = cg_update(MUGPU_FP64, fuse ? nullptr : d_x, d_r, d_p, d_Ap, c->scal, n, c->partials,
+                    c->ctl);
+    return e;
+  };
+  // On small cells every kernel of a chunk takes a few microseconds, and the
+  // host's launches, not the device, set the pace: the chunk is captured once
+  // into a hipGraph (per vectors, chunk length and first-iteration form; every
+  // other pointer the kernels take belongs to the cell) and replayed.  Not
+  // while kernels are timed one by one, which needs events between them.
+  const bool graph = g_cg_graphs && n <= kCgGraphMaxValues && g_timing_class < 0;
+  if (graph) {
+    hipGraphExec_t ex = nullptr;
+    const int fu = update_p ? 1 : 0, fx = pend_here ? 1 : 0;
+    for (auto& g : c->graphs)
+      if (g.x == d_x && g.r == d_r && g.p == d_p && g.Ap == d_Ap && g.steps == max_steps &&
+          g.first_update_p == fu && g.first_x == fx)
+        ex = g.exec;
+    if (!ex) {
+      hipGraph_t gr = nullptr;
+      MUGPU_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+      set_capturing(true);
+      for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) rc = queue(i);
+      set_capturing(false);
+      const hipError_t e = hipStreamEndCapture(g_stream, &gr);
+      if (rc == MUGPU_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+      if (rc == MUGPU_OK) {
+        const hipError_t ei = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) rc = hip_fail(ei, "hipGraphInstantiate", __FILE__, __LINE__);
+      }
+      if (gr) (void)hipGraphDestroy(gr);
+      if (rc == MUGPU_OK) {
+        if (c->graphs.size() >= kCgGraphsPerCell) {
+          (void)hipGraphExecDestroy(c->graphs.front().exec);
+          c->graphs.erase(c->graphs.begin());
+        }
+        c->graphs.push_back({d_x, d_r, d_p, d_Ap, max_steps, fu, fx, ex});
+      }
+    }
+    if (rc == MUGPU_OK && hipGraphLaunch(ex, g_stream) != hipSuccess)
+      rc = hip_fail(hipGetLastError(), "hipGraphLaunch", __FILE__, __LINE__);
+  } else {
+    for (int i = 0; i < max_steps && rc == MUGPU_OK; ++i) {
+      marks.push_back(timing_mark());
+      rc = queue(i);
+    }
   }
   c->plan.active = nullptr;
   MUGPU_TRY(rc);
@@ -1305,26 +1513,12 @@ extern "C" int mugpu_cg_steps(mugpu_cell c, int update_p, void* d_x, void* d_r, 
 }
 
 // ==================== mixed precision CG with FP64 refinement ===================
-extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, double tol,
-                                    int maxiter, double inner_tol, int64_t* iterations,
-                                    int* refinements, double* rel_residual) {
-  if (!c) return MUGPU_EARG;
-  if (c->prec != MUGPU_FP32 || !c->tan32) {
-    set_error("the mixed-precision solve needs a cell created with MUGPU_FP32 and an evaluated tangent");
-    return MUGPU_ERUNTIME;
-  }
+// Body of the mixed-precision solve (mugpu_cg_solve_mixed) over the work
+// vectors v; allocates nothing.
+int solve_mixed(mugpu_cell c, const MixedVecs& v, const void* d_rhs, void* d_x, double tol,
+                int maxiter, double inner_tol, int64_t* iterations, int* refinements,
+                double* rel_residual) {
   const int64_t n = c->N * c->nc;
-  if (!c->x32) {
-    bool ok = hipMalloc((void**)&c->x32, sizeof(float) * n) == hipSuccess &&
-              hipMalloc((void**)&c->r32, sizeof(float) * n) == hipSuccess &&
-              hipMalloc((void**)&c->p32, sizeof(float) * n) == hipSuccess &&
-              hipMalloc((void**)&c->Ap32, sizeof(float) * n) == hipSuccess &&
-              hipMalloc((void**)&c->r64, sizeof(double) * n) == hipSuccess;
-    if (!ok) {
-      set_error("device allocation of the mixed-precision CG vectors failed");
-      return MUGPU_EMEMORY;
-    }
-  }
   MUGPU_HIP(hipMemsetAsync(d_x, 0, sizeof(double) * n, g_stream));
   MUGPU_TRY(dot_device(MUGPU_FP64, d_rhs, d_rhs, n, c->partials, c->scal, FIN_STORE_TMP1));
   MUGPU_TRY(sync_scalars(c->scal));
@@ -1338,13 +1532,13 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     // true residual with the FP64 operator: r64 = b - A x; at the start
     // x = 0, so A x = 0 exactly and the residual is b itself (read in place
     // of r64 below), |b|^2 bit for bit
-    const double* res64 = first ? (const double*)d_rhs : c->r64;
+    const double* res64 = first ? (const double*)d_rhs : v.r64;
     if (first) {
       rr = bnorm2;
       first = false;
     } else {
-      MUGPU_TRY(cell_operator64(c, d_x, c->r64));
-      MUGPU_TRY(residual_dot((const double*)d_rhs, c->r64, n, c->partials, c->scal,
+      MUGPU_TRY(cell_operator64(c, d_x, v.r64));
+      MUGPU_TRY(residual_dot((const double*)d_rhs, v.r64, n, c->partials, c->scal,
                              FIN_STORE_TMP1));
       MUGPU_TRY(sync_scalars(c->scal));
       rr = g_host_scal[SC_TMP1];
@@ -1354,7 +1548,7 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     // inner CG in FP32 on A d = r64 / |r64|
     const double s = 1.0 / std::sqrt(rr);
     // r32 = -r64 s, p32 = -r32, x32 = 0, r32.r32: one pass over r64
-    MUGPU_TRY(mixed_start(res64, s, c->r32, c->p32, c->x32, n, c->partials, c->scal,
+    MUGPU_TRY(mixed_start(res64, s, v.r32, v.p32, v.x32, n, c->partials, c->scal,
                           FIN_STORE_RDR));
     MUGPU_TRY(sync_scalars(c->scal));
     const double rdr0 = g_host_scal[SC_RDR];
@@ -1365,22 +1559,25 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     // clears ctl[CTL_ACTIVE], after which the remaining kernels of the chunk
     // return at once; the host reads the counters once per chunk
     constexpr int kChunk = 8;
-    MUGPU_TRY(ensure_ctl(c));
     const int maxit = (int)(maxiter - its);
-    MUGPU_TRY(set_ctl(c->ctl, maxit, c->scal, tol2i));
+    MUGPU_TRY(set_ctl(v.ctl, maxit, c->scal, tol2i));
     int launched = 0;
     int rc = MUGPU_OK;
-    std::vector<int64_t> marks;
+    // timing marks of the current chunk's iterations: the loop stops inside
+    // the last chunk queued, so only its marks are needed
+    int64_t marks[kChunk];
+    int chunk0 = 0;
     // x updates fused into the next r2c (compile-time-length last axis)
     const bool fuse = c->plan.fastax[c->dim - 1];
-    c->plan.active = c->ctl;
+    c->plan.active = v.ctl;
     while (rc == MUGPU_OK) {
+      chunk0 = launched;
       for (int j = 0; j < kChunk && launched < maxit && rc == MUGPU_OK; ++j, ++launched) {
-        marks.push_back(timing_mark());
-        rc = inner_iteration(c, launched == 0, fuse, n);
+        marks[j] = timing_mark();
+        rc = inner_iteration(c, v, launched == 0, fuse, n);
       }
       if (rc != MUGPU_OK) break;
-      rc = sync_ctl(c->ctl, c->scal);
+      rc = sync_ctl(v.ctl, c->scal);
       if (rc != MUGPU_OK || !g_host_ctl[CTL_ACTIVE] || launched >= maxit) break;
     }
     c->plan.active = nullptr;
@@ -1389,12 +1586,12 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     // (the iteration with a NaN r.r, not counted, did)
     {
       const int ran = g_host_ctl[CTL_ITER] + (g_host_scal[SC_NEWRDR] == g_host_scal[SC_NEWRDR] ? 0 : 1);
-      if (ran < (int)marks.size()) timing_void(marks[ran]);
+      if (ran >= chunk0 && ran < launched) timing_void(marks[ran - chunk0]);
     }
     its += g_host_ctl[CTL_ITER];
     // x += |r64| d, with the last iteration's x update (alpha and p of the
     // last iteration run) folded in when it is pending
-    MUGPU_TRY(add_scaled_convert(d_x, c->x32, 1.0 / s, n, fuse ? c->p32 : nullptr, c->scal));
+    MUGPU_TRY(add_scaled_convert(d_x, v.x32, 1.0 / s, n, fuse ? v.p32 : nullptr, c->scal));
   }
 
   if (iterations) *iterations = its;
@@ -1406,4 +1603,31 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
     return MUGPU_ECONVERGENCE;
   }
   return MUGPU_OK;
+}
+
+int check_mixed_cell(mugpu_cell c) {
+  if (!c) return MUGPU_EARG;
+  if (c->prec != MUGPU_FP32 || !c->tan32) {
+    set_error("the mixed-precision solve needs a cell created with MUGPU_FP32 and an evaluated tangent");
+    return MUGPU_ERUNTIME;
+  }
+  return MUGPU_OK;
+}
+
+// ==================== mixed precision CG with FP64 refinement ===================
+extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, double tol,
+                                    int maxiter, double inner_tol, int64_t* iterations,
+                                    int* refinements, double* rel_residual) {
+  MUGPU_TRY(check_mixed_cell(c));
+  const int64_t n = c->N * c->nc;
+  MUGPU_TRY(ensure_mixed(c, n));
+  MixedVecs v;
+  v.x32 = c->x32;
+  v.r32 = c->r32;
+  v.p32 = c->p32;
+  v.Ap32 = c->Ap32;
+  v.r64 = c->r64;
+  v.ctl = c->ctl;
+  return solve_mixed(c, v, d_rhs, d_x, tol, maxiter, inner_tol, iterations, refinements,
+                     rel_residual);
 }

@@ -230,6 +230,34 @@ __global__ __launch_bounds__(kRT) void k_copy(const double2* __restrict__ s,
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) d[i] = s[i];
 }
 
+// Streaming copy for the bandwidth reference: every lane keeps kWide 16-byte
+// loads in flight (all issued before the first store, from clamped
+// addresses so that none waits behind a branch), optionally with
+// non-temporal hints on the loads and stores.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kWide = 4;
+template <bool NT>
+__global__ __launch_bounds__(kRT) void k_copy_wide(const u32x4* __restrict__ s,
+                                                   u32x4* __restrict__ d, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kRT * kWide;
+  for (int64_t i0 = (int64_t)blockIdx.x * kRT * kWide + threadIdx.x; i0 < n; i0 += stride) {
+    u32x4 v[kWide];
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) {
+      const int64_t i = i0 + (int64_t)k * kRT < n ? i0 + (int64_t)k * kRT : n - 1;
+      v[k] = NT ? __builtin_nontemporal_load(s + i) : s[i];
+    }
+#pragma unroll
+    for (int k = 0; k < kWide; ++k) {
+      const int64_t i = i0 + (int64_t)k * kRT;
+      if (i < n) {
+        if (NT) __builtin_nontemporal_store(v[k], d + i);
+        else d[i] = v[k];
+      }
+    }
+  }
+}
+
 // ---- batched inner CG: one RVE per blockIdx (finalize) or blockIdx.z ----
 // same arithmetic and reduction order as k_finalize / k_cg_update, so a slot
 // computes exactly what the unbatched FP32 inner CG computes
@@ -326,6 +354,21 @@ int copy_device(void* dst, const void* src, size_t bytes) {
   if (g < 1) g = 1;
   hipLaunchKernelGGL(k_copy, dim3((unsigned)g), dim3(kRT), 0, stream(), (const double2*)src,
                      (double2*)dst, n);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
+
+int copy_device_wide(void* dst, const void* src, size_t bytes, bool nontemporal, int grid) {
+  const int64_t n = (int64_t)(bytes / 16);
+  if (n < 1) return MUGPU_OK;
+  const int64_t need = (n + (int64_t)kRT * kWide - 1) / ((int64_t)kRT * kWide);
+  const unsigned g = (unsigned)(need < grid ? need : grid);
+  if (nontemporal)
+    hipLaunchKernelGGL(k_copy_wide<true>, dim3(g), dim3(kRT), 0, stream(), (const u32x4*)src,
+                       (u32x4*)dst, n);
+  else
+    hipLaunchKernelGGL(k_copy_wide<false>, dim3(g), dim3(kRT), 0, stream(), (const u32x4*)src,
+                       (u32x4*)dst, n);
   MUGPU_HIP(hipGetLastError());
   return MUGPU_OK;
 }

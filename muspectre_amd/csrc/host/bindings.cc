@@ -75,6 +75,15 @@ Precision precision_from(const py::object& p) {
 }
 
 template <Dim_t Dim>
+int64_t pixel_index(Cell& cell, const Ccoord_t<Dim>& pix);
+
+// pixels of a material of no cell are numbered in the order they come
+template <Dim_t Dim>
+int64_t material_pixel(MaterialBase& mat, const Ccoord_t<Dim>& pix) {
+  return mat.has_cell() ? pixel_index<Dim>(mat.get_cell(), pix) : (int64_t)mat.size();
+}
+
+template <Dim_t Dim>
 int64_t pixel_index(Cell& cell, const Ccoord_t<Dim>& pix) {
   const auto& res = cell.get_resolutions();
   int64_t i = 0;
@@ -94,12 +103,186 @@ void add_eigen_solver(py::module& m, const char* name) {
       .def("name", &Solver::get_name);
 }
 
-template <Dim_t Dim, int Fam>
-struct MaterialT : MaterialBase {
-  MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H, int fam = Fam)
-      : MaterialBase(c, n, fam, E, nu, tau, H) {}
-  MaterialT(Cell& c, const std::string& n, const Matrix& C_voigt) : MaterialBase(c, n, C_voigt) {}
+// MaterialBase<dim, dim> (the python Material_2d / Material_3d,
+// bind_py_material.cc:158-180), base of the per-family classes
+template <Dim_t Dim>
+struct MaterialDim : MaterialBase {
+  using MaterialBase::MaterialBase;
 };
+
+template <Dim_t Dim, int Fam>
+struct MaterialT : MaterialDim<Dim> {
+  MaterialT(Cell& c, const std::string& n, Real E, Real nu, Real tau, Real H, int fam = Fam)
+      : MaterialDim<Dim>(c, n, fam, E, nu, tau, H) {}
+  MaterialT(Cell& c, const std::string& n, const Matrix& C_voigt)
+      : MaterialDim<Dim>(c, n, C_voigt) {}
+  explicit MaterialT(const std::string& n) : MaterialDim<Dim>(n, Fam, Dim) {}
+};
+
+// ---- a material's internal field collection (bind_py_field_collection.cc) ----
+// LocalFieldCollection of a material: names, typed access; every field of
+// this build's materials holds Real values, so the Int, Uint and Complex
+// getters raise as the reference's get_typed_field does for a field of
+// another type (field_collection_base.hh:282-293)
+// (one C++ type per dimension: the 2D and 3D python classes are distinct)
+template <Dim_t Dim>
+struct FieldCollectionView {
+  MaterialBase* mat;
+};
+template <Dim_t Dim>
+struct FieldView {
+  MaterialBase* mat;
+  std::string name;
+};
+template <Dim_t Dim>
+struct StateFieldView {
+  MaterialBase* mat;
+  std::string prefix;
+};
+
+template <Dim_t Dim>
+py::array field_array(const FieldView<Dim>& f) {
+  Dim_t nb = 0;
+  auto v = f.mat->field_values(f.name, nb);
+  return field_to(v, nb, (int64_t)f.mat->size());
+}
+
+[[noreturn]] void wrong_type(MaterialBase& m, const std::string& name, const char* type) {
+  (void)m.field_values(name, *std::make_unique<Dim_t>());  // raises for unknown names
+  throw std::runtime_error("Field '" + name + "' is of type d, but should be of type " + type);
+}
+
+template <Dim_t Dim>
+void bind_field_collection(py::module& m) {
+  const std::string d = std::to_string(Dim);
+  py::class_<FieldView<Dim>>(m, ("LocalFieldReal_" + d).c_str())
+      .def_property_readonly("name", [](const FieldView<Dim>& f) { return f.name; }, "field name")
+      .def_property_readonly(
+          "collection",
+          py::cpp_function([](const FieldView<Dim>& f) { return FieldCollectionView<Dim>{f.mat}; },
+                           py::keep_alive<0, 1>()),
+          "Collection containing this field")
+      .def_property_readonly("nb_components",
+                             [](const FieldView<Dim>& f) {
+                               Dim_t nb = 0;
+                               f.mat->field_values(f.name, nb);
+                               return nb;
+                             },
+                             "number of scalars stored per pixel in this field")
+      .def_property_readonly("stored_type", [](const FieldView<Dim>&) { return std::string("d"); },
+                             "fundamental type of scalars stored in this field")
+      .def_property_readonly("size", [](const FieldView<Dim>& f) { return f.mat->size(); },
+                             "number of pixels in this field")
+      .def("set_zero",
+           [](FieldView<Dim>& f) {
+             Dim_t nb = 0;
+             auto v = f.mat->field_values(f.name, nb);
+             f.mat->set_field_values(f.name, std::vector<Real>(v.size(), 0.0));
+           },
+           "Set all components in the field to zero")
+      .def_property(
+          "array", [](const FieldView<Dim>& f) { return field_array(f); },
+          [](FieldView<Dim>& f, farr a) {
+            Dim_t nb = 0;
+            f.mat->field_values(f.name, nb);
+            f.mat->set_field_values(f.name, field_from(a, nb, (int64_t)f.mat->size()));
+          },
+          "array of stored data")
+      .def_property(
+          "vector",
+          [](const FieldView<Dim>& f) {
+            Dim_t nb = 0;
+            auto v = f.mat->field_values(f.name, nb);
+            return field_to(v, (int64_t)v.size(), 1);
+          },
+          [](FieldView<Dim>& f, farr a) {
+            Dim_t nb = 0;
+            auto v = f.mat->field_values(f.name, nb);
+            f.mat->set_field_values(f.name, field_from(a, (int64_t)v.size(), 1));
+          },
+          "flattened array of stored data");
+  py::class_<StateFieldView<Dim>>(m, ("LocalStateFieldReal_" + d).c_str())
+      .def_property_readonly("prefix", [](const StateFieldView<Dim>& f) { return f.prefix; },
+                             "state field prefix")
+      .def_property_readonly(
+          "collection",
+          py::cpp_function([](const StateFieldView<Dim>& f) { return FieldCollectionView<Dim>{f.mat}; },
+                           py::keep_alive<0, 1>()),
+          "Collection containing this field")
+      .def_property_readonly("nb_memory", [](const StateFieldView<Dim>&) { return 1; },
+                             "number of old states stored")
+      .def_property_readonly("stored_type", [](const StateFieldView<Dim>&) { return std::string("d"); },
+                             "fundamental type of scalars stored in this field")
+      // statefield.hh:245-269: current is sub-field indices[0], old indices[1]
+      .def("get_current_field",
+           [](const StateFieldView<Dim>& f) {
+             const int cur = f.mat->current_sub_field(f.prefix);
+             return FieldView<Dim>{f.mat, f.prefix + ", sub_field index " + std::to_string(cur)};
+           },
+           "returns the current field value", py::keep_alive<0, 1>())
+      .def("get_old_field",
+           [](const StateFieldView<Dim>& f, size_t nb_steps_ago) {
+             if (nb_steps_ago != 1) throw py::index_error("array::at");
+             const int cur = f.mat->current_sub_field(f.prefix);
+             return FieldView<Dim>{f.mat, f.prefix + ", sub_field index " + std::to_string(1 - cur)};
+           },
+           "nb_steps_ago"_a = 1, "returns the value this field held 'nb_steps_ago' steps ago",
+           py::keep_alive<0, 1>());
+  auto get_field = [](FieldCollectionView<Dim>& c, const std::string& n) {
+    Dim_t nb = 0;
+    c.mat->field_values(n, nb);  // raises for a name the collection does not hold
+    return FieldView<Dim>{c.mat, n};
+  };
+  auto get_state = [](FieldCollectionView<Dim>& c, const std::string& n) {
+    const auto names = c.mat->statefield_names();
+    if (std::find(names.begin(), names.end(), n) == names.end())
+      throw std::runtime_error("The state field named '" + n + "' does not exist in the material '" +
+                               c.mat->get_name() + "'");
+    return StateFieldView<Dim>{c.mat, n};
+  };
+  py::class_<FieldCollectionView<Dim>>(m, ("_LocalFieldCollection_" + d + "d").c_str())
+      .def("get_real_field", get_field, "unique_name"_a, py::keep_alive<0, 1>())
+      .def("get_int_field",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> FieldView<Dim> { wrong_type(*c.mat, n, "i"); },
+           "unique_name"_a)
+      .def("get_uint_field",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> FieldView<Dim> { wrong_type(*c.mat, n, "j"); },
+           "unique_name"_a)
+      .def("get_complex_field",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> FieldView<Dim> {
+             wrong_type(*c.mat, n, "St7complexIdE");
+           },
+           "unique_name"_a)
+      .def("get_real_statefield", get_state, "unique_name"_a, py::keep_alive<0, 1>())
+      .def("get_int_statefield",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> StateFieldView<Dim> {
+             throw std::runtime_error("State field '" + n + "' does not hold values of type i");
+           },
+           "unique_name"_a)
+      .def("get_uint_statefield",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> StateFieldView<Dim> {
+             throw std::runtime_error("State field '" + n + "' does not hold values of type j");
+           },
+           "unique_name"_a)
+      .def("get_complex_statefield",
+           [](FieldCollectionView<Dim>& c, const std::string& n) -> StateFieldView<Dim> {
+             throw std::runtime_error("State field '" + n +
+                                      "' does not hold values of type St7complexIdE");
+           },
+           "unique_name"_a)
+      .def_property_readonly("field_names",
+                             [](FieldCollectionView<Dim>& c) { return c.mat->field_names(); },
+                             "returns the names of all fields in this collection")
+      .def_property_readonly("statefield_names",
+                             [](FieldCollectionView<Dim>& c) { return c.mat->statefield_names(); },
+                             "returns the names of all state fields in this collection");
+  py::class_<MaterialDim<Dim>, MaterialBase>(m, ("Material_" + d + "d").c_str())
+      .def_property_readonly(
+          "collection",
+          py::cpp_function([](MaterialDim<Dim>& mat) { return FieldCollectionView<Dim>{&mat}; },
+                           py::keep_alive<0, 1>()),
+          "returns the field collection containing internal fields of this material");
+}
 
 template <Dim_t Dim, int Fam>
 MaterialT<Dim, Fam>& attach(CellBase<Dim, Dim>& sys, MaterialT<Dim, Fam>* m) {
@@ -111,7 +294,11 @@ void bind_material(py::module& m, const std::string& stem) {
   using Mat = MaterialT<Dim, Fam>;
   using Sys = CellBase<Dim, Dim>;
   const std::string name = stem + "_" + std::to_string(Dim) + "d";
-  py::class_<Mat, MaterialBase> cls(m, name.c_str());
+  py::class_<Mat, MaterialDim<Dim>> cls(m, name.c_str());
+  if (Fam == MUGPU_LE3 || Fam == MUGPU_LE4) {
+    // bind_py_material.cc:115,141: a material of no cell
+    cls.def(py::init<const std::string&>(), "name"_a);
+  }
   if (Fam == MUGPU_LE1 || Fam == MUGPU_LE2) {
     // bind_py_material.cc:66-71
     cls.def_static(
@@ -166,21 +353,21 @@ void bind_material(py::module& m, const std::string& stem) {
           std::vector<Real> p(Dim * Dim);
           for (int i = 0; i < Dim; ++i)
             for (int j = 0; j < Dim; ++j) p[i + Dim * j] = eig.at(i, j);
-          mat.add_pixel_index_with(pixel_index<Dim>(mat.get_cell(), pix), p);
+          mat.add_pixel_index_with(material_pixel<Dim>(mat, pix), p);
         },
         "pixel"_a, "eigenstrain"_a);
   } else if (Fam == MUGPU_LE3 || Fam == MUGPU_LE4) {
     cls.def(
         "add_pixel",
         [](Mat& mat, Ccoord_t<Dim> pix, Real E, Real nu) {
-          mat.add_pixel_index_with(pixel_index<Dim>(mat.get_cell(), pix), {E, nu});
+          mat.add_pixel_index_with(material_pixel<Dim>(mat, pix), {E, nu});
         },
         "pixel"_a, "Young"_a, "Poisson"_a);
   } else {
     cls.def(
         "add_pixel",
         [](Mat& mat, Ccoord_t<Dim> pix) {
-          mat.add_pixel_index(pixel_index<Dim>(mat.get_cell(), pix));
+          mat.add_pixel_index(material_pixel<Dim>(mat, pix));
         },
         "pixel"_a);
   }
@@ -271,6 +458,9 @@ void bind_cell(py::module& m) {
           },
           "strain"_a)
       .def("save_history_variables", &Sys::save_history_variables)
+      .def("get_projection",
+           [](Sys& s) { return CellProjection<Dim>(s); },
+           "the projection of the cell (CellBase::get_projection)", py::keep_alive<0, 1>())
       .def("get_subdomain_resolutions", &Sys::get_subdomain_resolutions)
       .def("get_subdomain_locations", &Sys::get_subdomain_locations)
       .def("get_domain_resolutions", &Sys::get_domain_resolutions)
@@ -339,18 +529,9 @@ void bind_engine(py::module& m, const std::string& name) {
 }
 
 template <class Proj, Dim_t Dim>
-void bind_projection(py::module& m, const std::string& stem) {
-  const std::string name = stem + "_" + std::to_string(Dim) + "d";
+void bind_projection_methods(py::class_<Proj>& cls) {
   const int nc = Dim * Dim;
-  py::class_<Proj>(m, name.c_str())
-      .def(py::init([](Ccoord_t<Dim> res, Rcoord_t<Dim> lengths, const std::string& fft) {
-             if (fft != "fftw" && fft != "hip")
-               throw std::runtime_error("Unknown FFT engine '" + fft + "' specified.");
-             std::unique_ptr<HIPEngine<Dim>> eng(new HIPEngine<Dim>(res, Dim * Dim));
-             return std::unique_ptr<Proj>(new Proj(std::move(eng), lengths));
-           }),
-           "resolutions"_a, "lengths"_a, "fft"_a = "fftw")
-      .def("initialise", &Proj::initialise, "flags"_a = FFT_PlanFlags::estimate,
+  cls.def("initialise", &Proj::initialise, "flags"_a = FFT_PlanFlags::estimate,
            "initialises the fft engine (plan the transform)")
       .def("apply_projection",
            [nc](Proj& p, farr v) {
@@ -373,6 +554,20 @@ void bind_projection(py::module& m, const std::string& stem) {
       .def("get_domain_resolutions", &Proj::get_domain_resolutions)
       // bind_py_projections.cc:181 binds get_domain_lengths to the resolutions
       .def("get_domain_lengths", &Proj::get_domain_resolutions);
+}
+
+template <class Proj, Dim_t Dim>
+void bind_projection(py::module& m, const std::string& stem) {
+  const std::string name = stem + "_" + std::to_string(Dim) + "d";
+  py::class_<Proj> cls(m, name.c_str());
+  cls.def(py::init([](Ccoord_t<Dim> res, Rcoord_t<Dim> lengths, const std::string& fft) {
+            if (fft != "fftw" && fft != "hip")
+              throw std::runtime_error("Unknown FFT engine '" + fft + "' specified.");
+            std::unique_ptr<HIPEngine<Dim>> eng(new HIPEngine<Dim>(res, Dim * Dim));
+            return std::unique_ptr<Proj>(new Proj(std::move(eng), lengths));
+          }),
+          "resolutions"_a, "lengths"_a, "fft"_a = "fftw");
+  bind_projection_methods<Proj, Dim>(cls);
 }
 
 template <Dim_t Dim>
@@ -492,6 +687,8 @@ PYBIND11_MODULE(_muSpectre, mod) {
   py::class_<MaterialBase>(material, "MaterialBase")
       .def("get_name", &MaterialBase::get_name)
       .def("size", &MaterialBase::size);
+  bind_field_collection<2>(material);
+  bind_field_collection<3>(material);
   bind_material<2, MUGPU_LE1>(material, "MaterialLinearElastic1");
   bind_material<3, MUGPU_LE1>(material, "MaterialLinearElastic1");
   bind_material<2, MUGPU_LE2>(material, "MaterialLinearElastic2");
@@ -584,4 +781,11 @@ PYBIND11_MODULE(_muSpectre, mod) {
   bind_projection<ProjectionFiniteStrain<3, 3>, 3>(fft, "ProjectionFiniteStrain");
   bind_projection<ProjectionFiniteStrainFast<2, 2>, 2>(fft, "ProjectionFiniteStrainFast");
   bind_projection<ProjectionFiniteStrainFast<3, 3>, 3>(fft, "ProjectionFiniteStrainFast");
+  // the projection a cell returns from get_projection (bind_py_cell.cc:197-198)
+  {
+    py::class_<CellProjection<2>> c2(fft, "CellProjection_2d");
+    bind_projection_methods<CellProjection<2>, 2>(c2);
+    py::class_<CellProjection<3>> c3(fft, "CellProjection_3d");
+    bind_projection_methods<CellProjection<3>, 3>(c3);
+  }
 }

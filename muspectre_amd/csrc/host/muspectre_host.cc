@@ -337,6 +337,19 @@ std::vector<Real> Cell::get_globalised_internal_real_array(const std::string& na
   return out;
 }
 
+std::vector<Real> Cell::get_projection_operator() const {
+  const int nop = form == Formulation::finite_strain ? dim : nb_components * nb_components;
+  std::vector<Real> op((size_t)(fourier_size() * nop));
+  check(mugpu_cell_projection_operator(cell, op.data()));
+  return op;
+}
+
+int64_t Cell::fourier_size() const {
+  int64_t n = 1;
+  for (Dim_t d = 0; d < dim; ++d) n *= d == dim - 1 ? res[d] / 2 + 1 : res[d];
+  return n;
+}
+
 MaterialBase& Cell::add_material(std::unique_ptr<MaterialBase> mat) {
   materials.push_back(std::move(mat));
   return *materials.back();
@@ -393,14 +406,24 @@ template std::unique_ptr<CellBase<3, 3>> make_cell<3, 3>(Ccoord_t<3>, Rcoord_t<3
 
 // ---- materials -------------------------------------------------------------------
 MaterialBase::MaterialBase(Cell& c, std::string n, int fam, Real E, Real nu, Real tau_y0, Real H)
-    : cell(c), name(std::move(n)), family(fam) {
+    : cell(&c), name(std::move(n)), family(fam), dim(c.get_material_dim()) {
   check(mugpu_cell_add_material(c.handle(), fam, E, nu, tau_y0, H, &mat));
   const int nc = c.get_material_dim() * c.get_material_dim();
   nb_params = fam == MUGPU_LE2 ? nc : (fam == MUGPU_LE3 || fam == MUGPU_LE4) ? 2 : 0;
 }
 
+MaterialBase::MaterialBase(std::string n, int fam, Dim_t d)
+    : cell(nullptr), name(std::move(n)), family(fam), dim(d) {
+  nb_params = fam == MUGPU_LE2 ? d * d : (fam == MUGPU_LE3 || fam == MUGPU_LE4) ? 2 : 0;
+}
+
+Cell& MaterialBase::get_cell() {
+  if (!cell) throw std::runtime_error("the material '" + name + "' belongs to no cell");
+  return *cell;
+}
+
 MaterialBase::MaterialBase(Cell& c, std::string n, const Matrix& C_voigt)
-    : cell(c), name(std::move(n)), family(MUGPU_LEG), nb_params(0) {
+    : cell(&c), name(std::move(n)), family(MUGPU_LEG), dim(c.get_material_dim()), nb_params(0) {
   const Dim_t vs = c.get_material_dim() == 2 ? 3 : 6;
   if (C_voigt.rows != vs || C_voigt.cols != vs) {
     // the reference builds this message (material_linear_elastic_generic.cc:
@@ -436,9 +459,137 @@ void MaterialBase::add_pixel_index_with(int64_t index, const std::vector<Real>& 
 
 void MaterialBase::upload() {
   if (uploaded) return;
-  check(mugpu_material_add_pixels(cell.handle(), mat, (int64_t)pixels.size(), pixels.data(),
+  check(mugpu_material_add_pixels(get_cell().handle(), mat, (int64_t)pixels.size(), pixels.data(),
                                   nb_params ? params.data() : nullptr));
   uploaded = true;
+}
+
+namespace {
+// the reference's internal field names (material_linear_elastic2.cc:44,
+// material_linear_elastic3.cc:45, material_linear_elastic4.cc:47-50,
+// material_hyper_elasto_plastic1.cc:46-53)
+const char* const kEigen = "Eigenstrain";
+const char* const kLocalC = "local stiffness tensor";
+const char* const kLambda = "local first Lame constant";
+const char* const kMu = "local second Lame constant(shear modulus)";
+const char* const kStatePrefixes[3] = {"cumulated plastic flow \xce\xb5\xe2\x82\x9a",
+                                       "Previous placement gradient F\xe1\xb5\x97",
+                                       "Previous left Cauchy-Green deformation b\xe2\x82\x91\xe1\xb5\x97"};
+// names of a state field's sub-fields (statefield.hh:181-190)
+std::string sub_field_name(const std::string& prefix, int i) {
+  std::stringstream n;
+  n << prefix << ", sub_field index " << i;
+  return n.str();
+}
+bool is_hep(int family) { return family == MUGPU_HEP1 || family == MUGPU_HEP1_CT; }
+// Hooke::compute_lambda / compute_mu (materials_toolbox.hh:488-525,603-619)
+Real lame_lambda(Real E, Real nu) { return E * nu / ((1 + nu) * (1 - 2 * nu)); }
+Real lame_mu(Real E, Real nu) { return E / (2 * (1 + nu)); }
+}  // namespace
+
+std::vector<std::string> MaterialBase::field_names() const {
+  std::vector<std::string> out;
+  if (family == MUGPU_LE2) out.push_back(kEigen);
+  if (family == MUGPU_LE3) out.push_back(kLocalC);
+  if (family == MUGPU_LE4) {
+    out.push_back(kLambda);
+    out.push_back(kMu);
+  }
+  if (is_hep(family))
+    for (const char* p : kStatePrefixes)
+      for (int i = 0; i < 2; ++i) out.push_back(sub_field_name(p, i));
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+std::vector<std::string> MaterialBase::statefield_names() const {
+  std::vector<std::string> out;
+  if (is_hep(family))
+    for (const char* p : kStatePrefixes) out.push_back(p);
+  std::sort(out.begin(), out.end());
+  return out;
+}
+
+std::vector<Real> MaterialBase::field_values(const std::string& fname, Dim_t& nb_comp) const {
+  const size_t np = pixels.size();
+  const Dim_t nc = dim * dim;
+  std::vector<Real> out;
+  if (family == MUGPU_LE2 && fname == kEigen) {
+    nb_comp = nc;
+    return params;
+  }
+  if (family == MUGPU_LE3 && fname == kLocalC) {
+    // Hooke::compute_C (materials_toolbox.hh:639-642): lambda I (x) I + 2 mu I4S,
+    // entry (i, j, k, l) at i + d j + d^2 k + d^3 l
+    nb_comp = nc * nc;
+    out.resize(np * (size_t)(nc * nc));
+    for (size_t q = 0; q < np; ++q) {
+      const Real la = lame_lambda(params[2 * q], params[2 * q + 1]);
+      const Real mu = lame_mu(params[2 * q], params[2 * q + 1]);
+      Real* C = out.data() + q * (size_t)(nc * nc);
+      for (Dim_t l = 0; l < dim; ++l)
+        for (Dim_t k = 0; k < dim; ++k)
+          for (Dim_t j = 0; j < dim; ++j)
+            for (Dim_t i = 0; i < dim; ++i) {
+              const Real itrac = (i == j && k == l) ? 1.0 : 0.0;
+              const Real isym = 0.5 * (((i == k && j == l) ? 1.0 : 0.0) + ((i == l && j == k) ? 1.0 : 0.0));
+              C[(i + dim * j) + nc * (k + dim * l)] = la * itrac + 2 * mu * isym;
+            }
+    }
+    return out;
+  }
+  if (family == MUGPU_LE4 && (fname == kLambda || fname == kMu)) {
+    nb_comp = 1;
+    out.resize(np);
+    for (size_t q = 0; q < np; ++q)
+      out[q] = fname == kLambda ? lame_lambda(params[2 * q], params[2 * q + 1])
+                                : lame_mu(params[2 * q], params[2 * q + 1]);
+    return out;
+  }
+  if (is_hep(family)) {
+    for (const char* p : kStatePrefixes)
+      for (int i = 0; i < 2; ++i)
+        if (fname == sub_field_name(p, i)) {
+          if (!cell || !cell->is_initialised())
+            throw std::runtime_error("the field '" + fname +
+                                     "' holds values once the cell is initialised");
+          int k = 0, cur = 0;
+          check(mugpu_material_get_state(cell->handle(), mat, p, i, nullptr, &k, &cur));
+          out.resize(np * (size_t)k);
+          check(mugpu_material_get_state(cell->handle(), mat, p, i, out.data(), &k, &cur));
+          nb_comp = k;
+          return out;
+        }
+  }
+  throw std::runtime_error("The field named '" + fname + "' does not exist in the material '" +
+                           name + "'");
+}
+
+void MaterialBase::set_field_values(const std::string& fname, const std::vector<Real>& values) {
+  if (!(family == MUGPU_LE2 && fname == kEigen))
+    throw std::runtime_error("the field '" + fname + "' of this build's material '" + name +
+                             "' is derived from its parameters and cannot be assigned");
+  if (uploaded)
+    throw std::runtime_error("the field '" + fname +
+                             "' lives on the device once the cell is initialised");
+  if (values.size() != params.size())
+    throw std::runtime_error("wrong number of values for the field '" + fname + "'");
+  params = values;
+}
+
+int MaterialBase::current_sub_field(const std::string& prefix) const {
+  if (is_hep(family))
+    for (const char* p : kStatePrefixes)
+      if (prefix == p) {
+        if (!cell || !cell->is_initialised())
+          throw std::runtime_error("the state field '" + prefix +
+                                   "' holds values once the cell is initialised");
+        int k = 0, cur = 0;
+        check(mugpu_material_get_state(cell->handle(), mat, p, 0, nullptr, &k, &cur));
+        return cur;
+      }
+  throw std::runtime_error("The state field named '" + prefix +
+                           "' does not exist in the material '" + name + "'");
 }
 
 // ---- solvers -----------------------------------------------------------------------

@@ -264,6 +264,10 @@ class Cell {
   void apply_projection(DeviceVector& v);
   void save_history_variables();
   std::vector<Real> get_globalised_internal_real_array(const std::string& name, Dim_t& nb_comp);
+  // host copy of the operator of the cell's projection (as
+  // ProjectionBase::get_operator, projection_base.hh:144)
+  std::vector<Real> get_projection_operator() const;
+  int64_t fourier_size() const;
   MaterialBase& add_material(std::unique_ptr<MaterialBase> mat);
   const std::vector<std::unique_ptr<MaterialBase>>& get_materials() const { return materials; }
   mugpu_cell handle() const { return cell; }
@@ -299,6 +303,33 @@ class CellBase : public Cell {
   int64_t get_index(const Ccoord& c) const;
 };
 
+// The cell's own projection, as CellBase::get_projection returns it
+// (cell_base.hh:323; bind_py_cell.cc:197-198): the projection of the cell's
+// formulation on its grid (small strain: ProjectionSmallStrain, finite
+// strain: ProjectionFiniteStrainFast), acting through the cell's operator
+// (no second plan or work space).
+template <Dim_t DimS, Dim_t DimM = DimS>
+class CellProjection {
+ public:
+  explicit CellProjection(CellBase<DimS, DimM>& cell) : cell(cell) {}
+  void initialise(FFT_PlanFlags flags = FFT_PlanFlags::estimate) { cell.initialise(flags); }
+  void apply_projection(DeviceVector& field) { cell.apply_projection(field); }
+  std::vector<Real> get_operator() const { return cell.get_projection_operator(); }
+  Formulation get_formulation() const { return cell.get_formulation(); }
+  Ccoord_t<DimS> get_domain_resolutions() const { return cell.get_domain_resolutions(); }
+  Ccoord_t<DimS> get_subdomain_resolutions() const { return cell.get_domain_resolutions(); }
+  Ccoord_t<DimS> get_subdomain_locations() const { return Ccoord_t<DimS>{}; }
+  Rcoord_t<DimS> get_domain_lengths() const { return cell.get_domain_lengths(); }
+  std::array<Dim_t, 2> get_strain_shape() const { return {DimM, DimM}; }
+  Dim_t get_nb_components() const { return DimM * DimM; }
+  int64_t size() const { return cell.size(); }
+  int64_t fourier_size() const { return cell.fourier_size(); }
+  CellBase<DimS, DimM>& get_cell() { return cell; }
+
+ private:
+  CellBase<DimS, DimM>& cell;
+};
+
 // make_cell (src/cell/cell_factory.hh:93-105)
 template <Dim_t DimS, Dim_t DimM = DimS>
 std::unique_ptr<CellBase<DimS, DimM>> make_cell(Ccoord_t<DimS> res, Rcoord_t<DimS> lengths,
@@ -314,6 +345,11 @@ class MaterialBase {
   // MaterialLinearElasticGeneric (material_linear_elastic_generic.cc:40-65):
   // one stiffness in Voigt notation, vsize x vsize
   MaterialBase(Cell& cell, std::string name, const Matrix& C_voigt);
+  // a material of no cell, as the reference's MaterialLinearElastic3/4(name)
+  // constructors make one (bind_py_material.cc:115,141): it takes pixels
+  // (linear indices of a grid it does not know) and exposes its internal
+  // fields, and is never evaluated
+  MaterialBase(std::string name, int family, Dim_t dim);
   virtual ~MaterialBase() = default;
   const std::string& get_name() const { return name; }
   int get_family() const { return family; }
@@ -326,12 +362,34 @@ class MaterialBase {
   // hands pixels to the device cell (at Cell::initialise)
   void upload();
   int index() const { return mat; }
-  Cell& get_cell() { return cell; }
+  bool has_cell() const { return cell != nullptr; }
+  Cell& get_cell();
+  Dim_t get_dim() const { return dim; }
+  // The material's internal field collection (MaterialMuSpectre::
+  // internal_fields, material_muSpectre_base.hh; FieldCollectionBase,
+  // field_collection_base.hh:107-122, 282-293), with the reference's field
+  // names: LE2 "Eigenstrain", LE3 "local stiffness tensor", LE4 "local first
+  // Lame constant" and "local second Lame constant(shear modulus)", HEP1 the
+  // state fields "cumulated plastic flow εₚ", "Previous placement gradient
+  // Fᵗ", "Previous left Cauchy-Green deformation bₑᵗ", each two fields
+  // "<prefix>, sub_field index 0/1".  Names are sorted as the reference's
+  // std::map keys are.
+  std::vector<std::string> field_names() const;
+  std::vector<std::string> statefield_names() const;
+  // values of a real field, one row of nb_comp values per pixel of the
+  // material (its own pixel order); throws for a name it does not hold
+  std::vector<Real> field_values(const std::string& name, Dim_t& nb_comp) const;
+  // overwrites a field held on the host (LE2's eigenstrain) before the cell
+  // is initialised; the device copy is made at initialisation
+  void set_field_values(const std::string& name, const std::vector<Real>& values);
+  // the sub-field of a state field that is current (statefield.hh:245-257)
+  int current_sub_field(const std::string& prefix) const;
 
  protected:
-  Cell& cell;
+  Cell* cell;
   std::string name;
   int family;
+  Dim_t dim;
   int mat{-1};
   int nb_params;
   std::vector<int64_t> pixels;

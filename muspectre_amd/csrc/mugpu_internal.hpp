@@ -26,6 +26,18 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     if (rc_ != MUGPU_OK) return rc_; \
   } while (0)
 
+// Device allocation of the library's own buffers (cells, tangents, plans,
+// work vectors): a failed hipMalloc has its sticky error cleared (the next
+// launch check would report it otherwise), the field cache is handed back to
+// the device after the library stream's work, and the allocation is retried
+// once.  On failure *p is nullptr and MUGPU_EMEMORY is returned with `what`
+// in the message.
+int dev_alloc(void** p, size_t bytes, const char* what);
+template <typename T>
+inline int dev_alloc(T** p, size_t bytes, const char* what) {
+  return dev_alloc(reinterpret_cast<void**>(p), bytes, what);
+}
+
 hipStream_t stream();
 // route the library's launches to another stream (nullptr: the library
 // stream); used by the serving loop, which drives two streams from one thread

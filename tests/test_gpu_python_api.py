@@ -354,3 +354,97 @@ def test_eigen_solver_convergence_error(µ, solver_name):
     solver = getattr(µ.solvers, solver_name)(cell, 1e-8, 2, False)
     with pytest.raises(µ.ConvergenceError, match="has not converged"):
         µ.solvers.newton_cg(cell, np.array([[0, 0], [0, .03]]), solver, 1e-5, 2, 0)
+
+
+# ---- python_field_tests.py:40-113: a material's field collection ----------
+def test_field_collection_eigenstrain(µ):
+    res, lengths = [3, 3], [1.58, 5.87]
+    cell = µ.Cell(res, lengths, µ.Formulation.finite_strain)
+    mat = µ.material.MaterialLinearElastic2_2d.make(cell, "material", 210e9, .33)
+    eigen_strain = np.array([[.01, .02], [.03, -.01]])
+    for i, pixel in enumerate(cell):
+        mat.add_pixel(pixel, i / cell.size * eigen_strain)
+    cell.initialise()
+    assert isinstance(mat, µ.material.Material_2d)
+    collection = mat.collection
+    field_name = collection.field_names[0]
+    assert field_name == "Eigenstrain"
+    for getter in (collection.get_complex_field, collection.get_int_field,
+                   collection.get_uint_field):
+        with pytest.raises(Exception):
+            getter(field_name)
+    field = collection.get_real_field(field_name)
+    for i, row in enumerate(field.array.T):
+        assert np.linalg.norm(i / cell.size * eigen_strain - row.reshape(eigen_strain.shape).T) == 0
+    glob = cell.get_globalised_internal_real_array("Eigenstrain")
+    assert np.linalg.norm(field.array - glob) == 0
+
+
+def test_field_collection_hep1_state_vs_oracle(µ, oracle):
+    """HEP1's state fields through the collection: the current and old
+    sub-fields match the oracle's current and old values across a history
+    cycle (statefield.hh:245-326), and the current one is what
+    get_globalised_internal_real_array returns."""
+    res = [5, 7, 3]
+    N = int(np.prod(res))
+    rng = np.random.default_rng(8)
+    hard = rng.random(N) < 0.5
+    cell = µ.Cell(res, [1.0] * 3, µ.Formulation.finite_strain)
+    mk = µ.material.MaterialHyperElastoPlastic1_3d.make
+    soft = mk(cell, "soft", 70e9, .33, 200e6, 1e9)
+    hm = mk(cell, "hard", 210e9, .33, 800e6, 1e9)
+    idx = np.arange(N)
+    hm.add_pixels_by_index(idx[hard])
+    soft.add_pixels_by_index(idx[~hard])
+    oc = oracle.Cell(res, [1.0] * 3, oracle.FINITE_STRAIN)
+    a = oc.add_material(oracle.HEP1, 70e9, .33, 200e6, 1e9)
+    b = oc.add_material(oracle.HEP1, 210e9, .33, 800e6, 1e9)
+    for i in range(N):
+        oc.add_pixel(b if hard[i] else a, i)
+    cell.initialise()
+    oc.initialise()
+    col = hm.collection
+    assert col.statefield_names == sorted(["cumulated plastic flow εₚ", "Previous placement gradient Fᵗ",
+                                           "Previous left Cauchy-Green deformation bₑᵗ"])
+    assert len(col.field_names) == 6
+    names = {"eps_p": "cumulated plastic flow εₚ", "F_prev": "Previous placement gradient Fᵗ",
+             "be": "Previous left Cauchy-Green deformation bₑᵗ"}
+    for step in range(3):
+        F = np.eye(3).ravel(order="F") + 0.01 * (step + 1) * rng.standard_normal((N, 9))
+        cell.evaluate_stress_tangent(F.T)
+        oc.F[:] = F
+        oc.evaluate_stress_tangent()
+        for short, prefix in names.items():
+            sf = col.get_real_statefield(prefix)
+            cur = sf.get_current_field().array.T
+            old = sf.get_old_field().array.T
+            for which, val in ((0, cur), (1, old)):
+                ref = oc.get_internal(short, which)[hard]
+                assert np.abs(val - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (short, which)
+            glob = cell.get_globalised_internal_real_array(prefix).T[hard]
+            np.testing.assert_array_equal(glob, cur)
+        cell.save_history_variables()
+        oc.save_history()
+
+
+def test_cell_get_projection(µ, oracle):
+    """cell.get_projection() (bind_py_cell.cc:197-198): the projection of
+    the cell's formulation; apply_projection equals cell.project and the
+    operator equals the oracle's stored one."""
+    for form, kind in ((µ.Formulation.small_strain, oracle.PROJ_SMALL),
+                       (µ.Formulation.finite_strain, oracle.PROJ_FINITE_FAST)):
+        res, lengths = [5, 7, 9], [1.0, 1.3, 0.7]
+        cell = µ.Cell(res, lengths, form)
+        m = µ.material.MaterialLinearElastic1_3d.make(cell, "m", 70e9, .3)
+        for pixel in cell:
+            m.add_pixel(pixel)
+        cell.initialise()
+        proj = cell.get_projection()
+        assert proj.get_formulation() == form
+        assert list(proj.get_domain_resolutions()) == res
+        field = np.random.default_rng(2).standard_normal((9, int(np.prod(res))))
+        np.testing.assert_array_equal(proj.apply_projection(field), cell.project(field))
+        op = proj.get_operator()
+        ref = oracle.projection_operator(kind, res, lengths)
+        assert op.shape == ref.T.shape
+        assert np.abs(op - ref.T).max() <= 1e-14
