@@ -121,6 +121,10 @@ class Workload:
             self.n = args.size or 128
             self.res = [self.n] * 3
             self.mask = spheres_mask(self.n, 40, 2026)
+            # 40 spheres of radius U[4, 10] fill ~3 % of 128^3 (the configuration's
+            # "~30 %" does not follow from its own numbers); the actual fraction
+            # is reported
+            self.inclusion_fraction = float(self.mask.mean())
             self.form = "small"
             eye = np.eye(3)
             cases = []
@@ -262,13 +266,31 @@ def kernel_bytes(wl, kclass, prec):
 
 
 # ------------------------------------------------------------- CPU baseline
-def cpu_baseline(wl, budget_s):
+def host_info():
+    """CPU model and core counts of the host this runs on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _cpu_sample_cell(wl):
+    """The oracle cell of the CPU sample: the workload's material and load at
+    a reduced grid (at least 64^3 in 3D, so that fields and the stored tangent,
+    170 MB and more, do not sit in the CPU's caches), strain, stress and
+    tangent evaluated."""
     from oracle import oracle as O
-    O.set_threads(1)
-    if wl.name in ("c3", "c2"):
-        n = 48 if wl.name == "c3" else 64
-        res = [n] * 3
-    elif wl.name == "c5":
+    if wl.name in ("c3", "c2", "c5"):
         n = 64
         res = [n] * 3
     else:
@@ -285,11 +307,23 @@ def cpu_baseline(wl, budget_s):
             c.add_pixel(b if mask.flat[i] else a, i)
         c.initialise()
         c.F[:] = np.eye(3).ravel(order="F") + np.diag([0, 0, .005]).ravel(order="F")
+    elif wl.name == "c5":
+        rng = np.random.default_rng(2026)
+        E = np.exp(np.log(50e9) + 0.8 * rng.standard_normal(N))
+        nu = rng.uniform(0.15, 0.45, N)
+        c = O.Cell(res, [1.0] * 3, O.SMALL_STRAIN, allow_even=True)
+        m = c.add_material(O.LE4)
+        for i in range(N):
+            c.add_pixel(m, i, [E[i], nu[i]])
+        c.initialise()
+        e = np.zeros((3, 3))
+        e[0, 1] = e[1, 0] = 0.01
+        c.F[:] = e.ravel(order="F")
     else:
         mask = spheres_mask(n, 40, 2026) if dim == 3 else wl.mask
         c = O.Cell(res, [1.0] * dim, O.SMALL_STRAIN, allow_even=n % 2 == 0)
-        a = c.add_material(O.LE1, 100e9, .3)
-        b = c.add_material(O.LE1, 10e9, .3)
+        a = c.add_material(O.LE1, 100e9 if dim == 3 else 10e9, .3 if dim == 3 else .33)
+        b = c.add_material(O.LE1, 10e9 if dim == 3 else 70e9, .3 if dim == 3 else .33)
         for i in range(N):
             c.add_pixel(a if mask.flat[i] else b, i)
         c.initialise()
@@ -297,7 +331,11 @@ def cpu_baseline(wl, budget_s):
         e[0, 1] = e[1, 0] = 0.01
         c.F[:] = e.ravel(order="F")
     c.evaluate_stress_tangent()
-    rhs = -c.apply_projection(c.P.copy())
+    return c, res, N
+
+
+def _cpu_rate(O, c, rhs, N, threads, budget_s):
+    O.set_threads(threads)
     c.cg_fixed_iterations(rhs, 1)  # warm-up
     iters = 0
     t0 = time.perf_counter()
@@ -305,29 +343,45 @@ def cpu_baseline(wl, budget_s):
         c.cg_fixed_iterations(rhs, 1)
         iters += 1
         el = time.perf_counter() - t0
-        if el > budget_s or iters >= 200:
+        if el > budget_s or iters >= 400:
             break
-    return {"value": N * iters / el, "unit": "voxel-CG-iters/s", "cores": 1, "kind": "port",
-            "sample": "{} {} voxels, {} CG iterations of the oracle (FP64, stored {}x{} "
-                      "tangent and projection operator as in the reference), {:.1f} s"
-                      .format(wl.name, "x".join(map(str, res)), iters, dim * dim, dim * dim, el)}
+    return N * iters / el, iters, el
+
+
+def cpu_baseline(wl, budget_s):
+    """The oracle (CPU restatement of the reference's FFTW-double path) timed
+    on this host over a bounded sample of the workload: one thread (the
+    reference is single-threaded) and the host's CPU share (OMP_NUM_THREADS,
+    else every core), each for about budget_s seconds of CG iterations."""
+    from oracle import oracle as O
+    c, res, N = _cpu_sample_cell(wl)
+    rhs = -c.apply_projection(c.P.copy())
+    info = host_info()
+    share = int(info["omp_num_threads"] or info["affinity"] or 1)
+    share = max(1, min(share, info["affinity"] or share))
+    r1, i1, e1 = _cpu_rate(O, c, rhs, N, 1, budget_s)
+    rn, i_n, e_n = _cpu_rate(O, c, rhs, N, share, budget_s)
+    O.set_threads(1)
+    dim = len(res)
+    sample = ("{} {} voxels, CG iterations of the oracle (FP64, stored {}x{} tangent and "
+              "projection operator as in the reference): {} iterations in {:.1f} s on {} threads"
+              .format(wl.name, "x".join(map(str, res)), dim * dim, dim * dim, i_n, e_n, share))
+    return {"value": round(rn, 1), "unit": "voxel-CG-iters/s", "cores": share, "kind": "port",
+            "sample": sample,
+            "single_thread": {"value": round(r1, 1), "cores": 1,
+                              "sample": "{} iterations in {:.1f} s".format(i1, e1)},
+            "host": info}
 
 
 def copy_bandwidth(capi, nbytes=4 << 30, reps=5):
+    """GB/s (read + write) of device copies: the library's copy kernel, a
+    streaming copy with four 16-byte loads in flight per lane (plain and
+    non-temporal) and hipMemcpyAsync, each timed with HIP events."""
     lib = capi.load()
-    a, b = ctypes.c_void_p(), ctypes.c_void_p()
-    capi.check(lib.mugpu_alloc(nbytes, ctypes.byref(a)))
-    capi.check(lib.mugpu_alloc(nbytes, ctypes.byref(b)))
-    capi.check(lib.mugpu_copy(b, a, nbytes))
-    capi.check(lib.mugpu_synchronize())
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        capi.check(lib.mugpu_copy(b, a, nbytes))
-    capi.check(lib.mugpu_synchronize())
-    dt = (time.perf_counter() - t0) / reps
-    lib.mugpu_free(a)
-    lib.mugpu_free(b)
-    return 2 * nbytes / dt / 1e9
+    out = (ctypes.c_double * 4)()
+    capi.check(lib.mugpu_copy_bandwidth(nbytes, reps, out))
+    names = ["copy_kernel", "streaming", "streaming_nontemporal", "hipMemcpy"]
+    return {n: round(v, 1) for n, v in zip(names, out)}
 
 
 def traffic_from_profiles(wl, kname):
@@ -610,6 +664,9 @@ def main():
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic_from_profiles(wl, kernel_name(wl, kclass)),
+                "traffic_source": "committed rocprofv3 PMC figure (profiles/hbm_traffic.txt: "
+                                  "FETCH_SIZE + WRITE_SIZE per launch of this kernel, "
+                                  "calibrated on a copy of known size), not measured in this run",
                 "kernel": kernel_name(wl, kclass), "launches": klaunch,
                 "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": kb}
     fp32_error = None
@@ -619,6 +676,9 @@ def main():
             fp32_error["homogenised_stress_rel"], fp32_error["strain_field_rel"]),
             file=sys.stderr, flush=True)
     copy_gbs = copy_bandwidth(_capi)
+    achievable = max(copy_gbs.values())
+    if roof is not None:
+        roof["frac_of_achievable"] = round(roof["achieved"] / achievable, 4)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, args.cpu_seconds)
@@ -640,7 +700,9 @@ def main():
                    "load_cases": len(wl.loads), "precision": wl.prec,
                    "newton_tol": wl.newton_tol, "cg_tol": wl.cg_tol,
                    "newton_maxiter": wl.newton_maxiter or None, "converged": wl.converged,
-                   "setup_s_per_step": round(setup / args.steps, 2)},
+                   "setup_s_per_step": round(setup / args.steps, 2),
+                   **({"inclusion_fraction": round(wl.inclusion_fraction, 4)}
+                      if hasattr(wl, "inclusion_fraction") else {})},
         "roofline": roof,
         "whole_iteration_roofline": {
             "bytes_per_voxel_iter": wl.B_alg,
@@ -648,7 +710,8 @@ def main():
             "frac": round(wl.B_alg * value / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
         "kernels_note": "untimed breakdown step with HIP events around every launch",
-        "copy_kernel_GBs": round(copy_gbs, 1),
+        "copy_bandwidth_GBs": copy_gbs,
+        "achievable_GBs": achievable,
         "fp32_error": fp32_error,
         "cpu_baseline": cpu,
     }
