@@ -308,6 +308,27 @@ int mugpu_cg_solve_mixed(mugpu_cell cell, const void *d_rhs, void *d_x,
                          int64_t *iterations, int *refinements,
                          double *rel_residual);
 
+/* ---- mixed-precision CG without device allocation (no reference
+ * counterpart) -------------------------------------------------------------
+ * For callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc): mugpu_cg_solve_mixed with the same
+ * arguments, results and errors, whose work vectors (four FP32 and one FP64
+ * vector of nb_pixels x nb_components values, plus the loop's control words)
+ * live in a fixed device array of MUGPU_NOALLOC_SCRATCH_BYTES instead of
+ * memory allocated on the cell.  The array is part of the library, one per
+ * GPU, held until the process ends; the call allocates nothing (the cell,
+ * the right-hand side and x are the caller's).  Cells with more than
+ * MUGPU_NOALLOC_MAX_VALUES values (e.g. 3D up to 64^3, 2D up to 512^2) are
+ * rejected with MUGPU_EARG.  Every call on a GPU uses the same array: calls
+ * that overlap on one GPU (from several threads) race; calls in sequence do
+ * not depend on each other. */
+#define MUGPU_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define MUGPU_NOALLOC_MAX_VALUES 2796000ll
+int mugpu_cg_solve_mixed_noalloc(mugpu_cell cell, const void *d_rhs, void *d_x,
+                                 double tol, int maxiter, double inner_tol,
+                                 int64_t *iterations, int *refinements,
+                                 double *rel_residual);
+
 /* ---- serving loop (no reference counterpart) ----------------------------
  * Many independent RVE solves of uneven resolution on one GPU.  Each request
  * is one small-strain load step of newton_cg (src/solver/solvers.cc:44-217)

@@ -89,6 +89,7 @@ SIGNATURES = [
     ("mugpu_cg_finish", _int, [_vp]),
     ("mugpu_cg_graphs", _int, [_int]),
     ("mugpu_cg_solve_mixed", _int, [_vp, _vp, _vp, _dbl, _int, _dbl, _i64p, _ip, _dp]),
+    ("mugpu_cg_solve_mixed_noalloc", _int, [_vp, _vp, _vp, _dbl, _int, _dbl, _i64p, _ip, _dp]),
     ("mugpu_timing_start", _int, [_int]),
     ("mugpu_timing_stop", _int, [_i64p, _dp]),
     ("mugpu_timing_class", _int, [_int, _i64p, _dp]),
@@ -105,6 +106,7 @@ SIGNATURES = [
     ("mugpu_copy", _int, [_vp, _vp, ctypes.c_size_t]),
     ("mugpu_copy_bandwidth", _int, [ctypes.c_size_t, _int, _dp]),
 ]
+NOALLOC_MAX_VALUES = 2796000  # MUGPU_NOALLOC_MAX_VALUES
 K_LAST_R2C, K_MIDDLE, K_AXIS0, K_LAST_C2R, K_MATERIAL, K_CG_UPDATE = range(6)
 K_ALL = 6
 

@@ -26,6 +26,18 @@ int* g_host_ctl = nullptr;       // pinned, CTL_COUNT words after the scalars
 double* g_dev_scratch = nullptr;  // partials + scalars for mugpu_dot
 }  // namespace
 
+// Work memory of mugpu_cg_solve_mixed_noalloc: a fixed array in device
+// memory, one copy per GPU the library runs on, held until the process ends.
+// Host code never uses the array's name as an address (on the host it is
+// only a stand-in for the symbol): mugpu_init looks up its device address
+// once per device (hipGetSymbolAddress) and every use goes through that.
+static __device__ __align__(256) unsigned char scratch[MUGPU_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+}  // namespace
+
 void set_error(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -355,6 +367,11 @@ extern "C" int mugpu_init(int device) {
                           hipHostMallocDefault));
   g_host_ctl = reinterpret_cast<int*>(g_host_scal + SC_COUNT);
   MUGPU_HIP(hipMalloc((void**)&g_dev_scratch, sizeof(double) * (kRedBlocks + SC_COUNT)));
+  if (device < kMaxDevices && !g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    MUGPU_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = ptr;
+  }
   g_device = device;
   return MUGPU_OK;
 }
@@ -1628,6 +1645,52 @@ extern "C" int mugpu_cg_solve_mixed(mugpu_cell c, const void* d_rhs, void* d_x, 
   v.Ap32 = c->Ap32;
   v.r64 = c->r64;
   v.ctl = c->ctl;
+  return solve_mixed(c, v, d_rhs, d_x, tol, maxiter, inner_tol, iterations, refinements,
+                     rel_residual);
+}
+
+// ============ mixed-precision CG without device allocation ============
+// The work vectors of solve_mixed carved out of the per-device `scratch`
+// array, 256-byte aligned each: control words, x32, r32, p32, Ap32 (FP32,
+// n values each) and r64 (FP64, n values).
+static size_t noalloc_bytes(int64_t n) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  return up(sizeof(int) * CTL_COUNT) + 4 * up(sizeof(float) * (size_t)n) + up(sizeof(double) * (size_t)n);
+}
+
+extern "C" int mugpu_cg_solve_mixed_noalloc(mugpu_cell c, const void* d_rhs, void* d_x, double tol,
+                                            int maxiter, double inner_tol, int64_t* iterations,
+                                            int* refinements, double* rel_residual) {
+  if (!c) return MUGPU_EARG;
+  const int64_t n = c->N * c->nc;
+  if (n > MUGPU_NOALLOC_MAX_VALUES || noalloc_bytes(n) > MUGPU_NOALLOC_SCRATCH_BYTES) {
+    set_error("mugpu_cg_solve_mixed_noalloc: the cell has %lld values (pixels x components); the "
+              "allocation-free solve holds at most %lld in its fixed work array of %lld bytes",
+              (long long)n, (long long)MUGPU_NOALLOC_MAX_VALUES,
+              (long long)MUGPU_NOALLOC_SCRATCH_BYTES);
+    return MUGPU_EARG;
+  }
+  // the library is initialised (the cell exists); the work array's address
+  // was looked up for this device by mugpu_init
+  if (g_device < 0 || g_device >= kMaxDevices || !g_scratch_ptr[g_device]) {
+    set_error("mugpu_cg_solve_mixed_noalloc: the work array of device %d has no address", g_device);
+    return MUGPU_ERUNTIME;
+  }
+  MUGPU_TRY(check_mixed_cell(c));
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[g_device]);
+  MixedVecs v;
+  v.ctl = reinterpret_cast<int*>(base);
+  base += up(sizeof(int) * CTL_COUNT);
+  float** f32[4] = {&v.x32, &v.r32, &v.p32, &v.Ap32};
+  for (float** f : f32) {
+    *f = reinterpret_cast<float*>(base);
+    base += up(sizeof(float) * (size_t)n);
+  }
+  v.r64 = reinterpret_cast<double*>(base);
+  // every vector is written before it is read (x32, r32, p32 by the inner
+  // start, Ap32 by the operator, r64 by the residual, ctl by set_ctl), so
+  // what an earlier call left in the array does not matter
   return solve_mixed(c, v, d_rhs, d_x, tol, maxiter, inner_tol, iterations, refinements,
                      rel_residual);
 }
