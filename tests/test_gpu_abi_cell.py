@@ -433,3 +433,43 @@ def test_generic_material_errors(capi):
     m = ctypes.c_int()
     with pytest.raises(capi.MugpuError):
         capi.check(gc.lib.mugpu_cell_add_material(gc.h, 7, 1.0, .3, 0.0, 0.0, ctypes.byref(m)))
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[6], CASES[7]], ids=lambda c: c[0])
+def test_cg_steps_graph_replay_equals_direct_launches(capi, case):
+    """On small cells mugpu_cg_steps replays each chunk from a captured
+    hipGraph.  Three solves in a row (the graphs replayed, with the solves'
+    own start, finish and host synchronisations between the replays) give
+    bit for bit the iterates of the same solves launched kernel by kernel,
+    and the iteration counts of the host step loop."""
+    name, res, L, form, f_in, f_out, Ein, Eout = case[:8]
+    even = case[8] if len(case) > 8 else 0
+    gc = GpuCell(capi, res, L, form, allow_even=even)
+    mask = _sphere_mask(res, min(res) / 3)
+    mi = gc.add_material(f_in, Ein, .3)
+    mo = gc.add_material(f_out, Eout, .3)
+    idx = np.arange(mask.size)
+    gc.add_pixels(mi, idx[mask.ravel()])
+    gc.add_pixels(mo, idx[~mask.ravel()])
+    capi.check(gc.lib.mugpu_cell_initialise(gc.h))
+    F = np.zeros((gc.N, gc.nc))
+    if form == 0:
+        F += np.eye(len(res)).ravel(order="F")
+    F[:, 1] += 0.01
+    gc.set_F(F)
+    capi.check(gc.lib.mugpu_cell_evaluate_stress_tangent(gc.h))
+    rhs = [np.random.default_rng(s).standard_normal((gc.N, gc.nc)) for s in (1, 2, 1)]
+    try:
+        capi.check(gc.lib.mugpu_cg_graphs(0))
+        direct = [_gpu_cg_chunked(gc, b, 1e-8, 400, 8) for b in rhs]
+        capi.check(gc.lib.mugpu_cg_graphs(1))
+        replay = [_gpu_cg_chunked(gc, b, 1e-8, 400, 8) for b in rhs]
+    finally:
+        capi.check(gc.lib.mugpu_cg_graphs(1))
+    for (xd, rd_, itd, _, _), (xg, rg, itg, _, _) in zip(direct, replay):
+        assert itg == itd
+        np.testing.assert_array_equal(xg, xd)
+        np.testing.assert_array_equal(rg, rd_)
+    ref_it = _gpu_cg(gc, rhs[0], 1e-8, 400)[1]
+    assert replay[0][2] == ref_it and replay[2][2] == ref_it
+    np.testing.assert_array_equal(replay[0][0], replay[2][0])
