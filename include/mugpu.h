@@ -429,12 +429,14 @@ int mugpu_fft_tune_pass(int dim, const int *resolutions, int precision, int pass
 /* streaming device copy of `bytes` (multiple of 16), the bandwidth
  * reference for the roofline and for the HBM counter calibration */
 int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);
-/* achievable device bandwidth: GB/s (bytes read + written) of four copies
- * of `bytes` between two fresh device blocks, each timed with HIP events over
- * `reps` launches after one untimed launch: gbs[0] mugpu_copy's kernel, gbs[1]
- * a streaming copy with four 16-byte loads in flight per lane, gbs[2] the
- * same with non-temporal loads and stores, gbs[3] hipMemcpyAsync device to
- * device */
+/* achievable device bandwidth: GB/s of six streams over `bytes` in two
+ * fresh device blocks, each timed with HIP events over `reps` launches after
+ * one untimed launch: copies (bytes read + written) gbs[0] mugpu_copy's
+ * kernel, gbs[1] a streaming copy with four 16-byte loads in flight per lane,
+ * gbs[2] the same with non-temporal loads and stores, gbs[3]
+ * hipMemcpyAsync device to device; gbs[4] a read-only stream (non-temporal
+ * loads, four in flight per lane), gbs[5] a write-only stream (non-temporal
+ * stores).  gbs holds 6 values. */
 int mugpu_copy_bandwidth(size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus

@@ -104,6 +104,8 @@ void timing_void(int64_t from) {
 
 int copy_device(void* dst, const void* src, size_t bytes);
 int copy_device_wide(void* dst, const void* src, size_t bytes, bool nontemporal, int grid);
+int read_device_wide(const void* src, size_t bytes, void* sink, int grid);
+int fill_device_wide(void* dst, size_t bytes, int grid);
 
 int run_tangent_product(int dim, int prec, int kind, const void* tan, const void* in, void* out,
                         int64_t npix);
@@ -737,16 +739,18 @@ extern "C" int mugpu_copy_bandwidth(size_t bytes, int reps, double* gbs) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) rc = MUGPU_EHIP;
   if (!rc && hipMemsetAsync(a, 1, bytes, g_stream) != hipSuccess) rc = MUGPU_EHIP;
-  for (int v = 0; v < 4 && !rc; ++v) {
+  for (int v = 0; v < 6 && !rc; ++v) {
     auto run = [&]() -> int {
       switch (v) {
         case 0: return copy_device(b, a, bytes);
         case 1: return copy_device_wide(b, a, bytes, false, cus * 8);
         case 2: return copy_device_wide(b, a, bytes, true, cus * 8);
-        default:
+        case 3:
           return hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, g_stream) == hipSuccess
                      ? MUGPU_OK
                      : MUGPU_EHIP;
+        case 4: return read_device_wide(a, bytes, b, cus * 8);
+        default: return fill_device_wide(b, bytes, cus * 8);
       }
     };
     rc = run();  // warm-up
@@ -756,7 +760,9 @@ extern "C" int mugpu_copy_bandwidth(size_t bytes, int reps, double* gbs) {
       rc = MUGPU_EHIP;
     float ms = 0.f;
     if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MUGPU_EHIP;
-    if (!rc) gbs[v] = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    // bytes moved: read and written for the copies, read or written once
+    // for the read-only and write-only streams
+    if (!rc) gbs[v] = (v < 4 ? 2.0 : 1.0) * (double)bytes * reps / (ms * 1e-3) / 1e9;
   }
   if (rc == MUGPU_EHIP) set_error("HIP error in the bandwidth measurement");
   (void)hipStreamSynchronize(g_stream);
