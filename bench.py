@@ -395,6 +395,30 @@ def copy_bandwidth(capi, nbytes=4 << 30, reps=5):
     return {n: round(v, 1) for n, v in zip(names, out)}
 
 
+def line_bandwidth(capi, wl, reps=3):
+    """GB/s (read + write) of the first-axis pass's HBM pattern alone, for
+    this workload's spectrum (n rows of plane k-points, nc components) with
+    tiles of 1, 2 and 4 k-points: the bandwidth that access pattern allows,
+    before any arithmetic."""
+    lib = capi.load()
+    n = wl.res[0]
+    if n % 16:
+        return None
+    dim = len(wl.res)
+    nc = dim * dim
+    herm = list(wl.res[:-1]) + [wl.res[-1] // 2 + 1]
+    plane = int(np.prod(herm)) // n
+    prec = capi.FP64 if wl.prec == "fp64" else capi.FP32
+    out = {}
+    for kp in (1, 2, 4):
+        if kp * nc * (n // 16) > 1024:
+            continue
+        g = ctypes.c_double()
+        capi.check(lib.mugpu_line_bandwidth(prec, n, plane, nc, kp, reps, ctypes.byref(g)))
+        out["kpts_per_tile_{}".format(kp)] = round(g.value, 1)
+    return out
+
+
 def kernel_read_write_bytes(wl, kclass, prec):
     """(read, written) algorithmic bytes of one launch (their sum is
     kernel_bytes)."""
@@ -744,6 +768,11 @@ def main():
         if rw:
             e["frac_of_stream_mix"] = round(achievable_time(rw, copy_gbs) /
                                             (e["avg_launch_us"] * 1e-6), 4)
+    line_gbs = line_bandwidth(_capi, wl)
+    if line_gbs and "axis0" in kernels and kernels["axis0"].get("achieved_GBs"):
+        kernels["axis0"]["line_pattern_GBs"] = line_gbs
+        kernels["axis0"]["frac_of_line_pattern"] = round(
+            kernels["axis0"]["achieved_GBs"] / max(line_gbs.values()), 4)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, args.cpu_seconds)

@@ -438,6 +438,16 @@ int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);
  * loads, four in flight per lane), gbs[5] a write-only stream (non-temporal
  * stores).  gbs holds 6 values. */
 int mugpu_copy_bandwidth(size_t bytes, int reps, double *gbs);
+/* bandwidth of the first-axis pass's HBM pattern alone (no reference
+ * counterpart; a roofline anchor for the fused first-axis kernel): a fresh
+ * device block of n rows (n a multiple of 16) of plane k-points with
+ * nb_components complex values each (precision MUGPU_FP64 or MUGPU_FP32) is
+ * read and written in place by tiles of kpts_per_tile k-points, every thread
+ * holding 16 values of one column, in the tile order of that kernel; *gbs =
+ * bytes read + written per second over `reps` launches after one untimed
+ * launch.  kpts_per_tile * nb_components * n / 16 <= 1024. */
+int mugpu_line_bandwidth(int precision, int n, int64_t plane, int nb_components,
+                         int kpts_per_tile, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

@@ -773,6 +773,40 @@ extern "C" int mugpu_copy_bandwidth(size_t bytes, int reps, double* gbs) {
   return rc;
 }
 
+extern "C" int mugpu_line_bandwidth(int precision, int n, int64_t plane, int nb_components,
+                                    int kpts_per_tile, int reps, double* gbs) {
+  MUGPU_TRY(ensure_init());
+  if (!gbs || reps < 1 || n < 16 || n % 16 || plane < 1 || nb_components < 1 ||
+      kpts_per_tile < 1 || (int64_t)kpts_per_tile * nb_components * (n / 16) > 1024 ||
+      (precision != MUGPU_FP64 && precision != MUGPU_FP32)) {
+    set_error("mugpu_line_bandwidth needs n a multiple of 16, reps >= 1 and at most 1024 threads "
+              "(kpts_per_tile * nb_components * n / 16)");
+    return MUGPU_EARG;
+  }
+  const size_t bytes = (size_t)n * plane * nb_components * (precision == MUGPU_FP64 ? 16 : 8);
+  void* a = nullptr;
+  MUGPU_TRY(dev_alloc(&a, bytes, "the line-pattern buffer"));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MUGPU_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = MUGPU_EHIP;
+  // 0x3f bytes: finite values in both precisions
+  if (!rc && hipMemsetAsync(a, 0x3f, bytes, g_stream) != hipSuccess) rc = MUGPU_EHIP;
+  if (!rc) rc = line_pattern(a, precision, n, plane, nb_components, kpts_per_tile);  // warm-up
+  if (!rc && hipEventRecord(e0, g_stream) != hipSuccess) rc = MUGPU_EHIP;
+  for (int i = 0; i < reps && !rc; ++i) rc = line_pattern(a, precision, n, plane, nb_components, kpts_per_tile);
+  if (!rc && (hipEventRecord(e1, g_stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+    rc = MUGPU_EHIP;
+  float ms = 0.f;
+  if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MUGPU_EHIP;
+  if (!rc) *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  if (rc == MUGPU_EHIP) set_error("HIP error in the line-pattern measurement");
+  (void)hipStreamSynchronize(g_stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  free_dev(a);
+  return rc;
+}
+
 // ============================== FFT engine ====================================
 extern "C" int mugpu_fft_create(int dim, const int* res, int nb_components, int precision,
                                 mugpu_fft* out) {

@@ -492,20 +492,14 @@ class SolverGMRESEigen : public SolverEigenBase {
  protected:
   bool run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) override;
   Uint restart{30};  // Eigen's GMRES and DGMRES default
-  // start from x0 = b instead of x0 = 0 (Eigen's DGMRES with the identity
-  // preconditioner)
-  bool start_from_rhs{false};
 };
 
 // Eigen's DGMRES with its default settings, which deflate no eigenvalues:
-// restarted GMRES(30) started from x0 = b (solvers_krylov.cc states the
-// correspondence)
+// the restarted GMRES(30) of SolverGMRESEigen (solvers_krylov.cc states the
+// correspondence and the one deliberate difference, the start x0 = 0)
 class SolverDGMRESEigen : public SolverGMRESEigen {
  public:
-  SolverDGMRESEigen(Cell& cell, Real tol, Uint maxiter, bool verbose = false)
-      : SolverGMRESEigen(cell, tol, maxiter, verbose) {
-    start_from_rhs = true;
-  }
+  using SolverGMRESEigen::SolverGMRESEigen;
   std::string get_name() const override { return "DGMRES"; }
 };
 

@@ -176,19 +176,21 @@ bool SolverBiCGSTABEigen::run(const DeviceVector& b, DeviceVector& x, Uint& iter
 //
 // DGMRES (Eigen 3.3, unsupported/Eigen/src/IterativeSolvers/DGMRES.h) is
 // bound by the reference with its defaults (solver_eigen.hh:93-100; nothing
-// in the reference calls setEigenv), and the default number of eigenvalues
-// to deflate is zero: DGMRES::dgmresCycle then never computes deflation
-// data and each cycle is the restarted GMRES(30) cycle above.  What remains
-// different is the start: DGMRES::dgmres replaces a zero initial guess by
-// the preconditioner applied to the right-hand side (x0 = b for the identity
-// preconditioner) and returns without iterating when |b - A x0| / |b| is
-// already below tol.  start_from_rhs selects that start.
+// in the reference calls setEigenv), and by default it deflates no
+// eigenvalues: DGMRES::dgmresCycle then computes no deflation data and each
+// cycle is the restarted GMRES(30) cycle below, which SolverDGMRESEigen
+// runs.  One difference is deliberate: Eigen's DGMRES replaces a zero
+// initial guess by the preconditioner applied to the right-hand side (x0 = b
+// for the identity preconditioner).  For the Newton systems of a cell, b is a
+// stress-sized residual and x a strain increment; with moduli in Pa, x0 = b
+// starts from a residual about |K| |b| and a relative tolerance of 1e-8 is
+// then below what FP64 can resolve.  Both solvers start from x0 = 0.
 bool SolverGMRESEigen::run(const DeviceVector& b, DeviceVector& x, Uint& iters, Real& error) {
   const int64_t n = b.size();
+  check(mugpu_memset_zero(x.data(), sizeof(double) * n));
   iters = 0;
-  const Real b_norm = std::sqrt(dot(b, b));
-  if (b_norm == 0) {
-    check(mugpu_memset_zero(x.data(), sizeof(double) * n));
+  const Real r0_norm = std::sqrt(dot(b, b));
+  if (r0_norm == 0) {
     error = 0;
     return true;
   }
@@ -197,21 +199,9 @@ bool SolverGMRESEigen::run(const DeviceVector& b, DeviceVector& x, Uint& iters, 
   V.reserve(m + 1);
   for (Uint i = 0; i <= m; ++i) V.emplace_back(n);
   DeviceVector r(n), w(n);
-  Real beta = b_norm;
-  if (start_from_rhs) {
-    scale_copy(1.0, b, x);
-    apply(x, w);
-    scale_copy(1.0, b, r);
-    axpby(-1.0, w, 1.0, r);
-    beta = std::sqrt(dot(r, r));
-    error = beta / b_norm;
-    if (error < tol || beta == 0) return std::isfinite(error);
-  } else {
-    check(mugpu_memset_zero(x.data(), sizeof(double) * n));
-    scale_copy(1.0, b, r);
-    error = 1.0;
-  }
-  const Real r0_norm = b_norm;
+  scale_copy(1.0, b, r);
+  Real beta = r0_norm;
+  error = 1.0;
   std::vector<Real> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
   auto h = [&](Uint i, Uint j) -> Real& { return H[(size_t)i * m + j]; };
   while (true) {

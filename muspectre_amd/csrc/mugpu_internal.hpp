@@ -185,6 +185,9 @@ int fast_r2c(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_c2r(FftPlan& p, int prec, const OpSpec& op, void* spec);
 int fast_strided(FftPlan& p, int prec, int ax, void* spec, int sign);
 int fast_axis0_project(FftPlan& p, int prec, const ProjParams& pj, void* spec);
+// the HBM pattern of the first-axis pass alone (n rows of plane k-points of nc
+// complex components, tiles of kp k-points), in place (fft_fast.hip)
+int line_pattern(void* data, int prec, int n, int64_t plane, int nc, int kp);
 // batched FP32 operator G:(K:p) with the p update and the p . Ap partials,
 // one RVE per slot (fft_fast.hip)
 int fast_batch_operator(FftPlan& p, const BatchSlot* d_slots, int nslots);
