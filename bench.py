@@ -604,6 +604,16 @@ def c4_main(args):
     st = srv.stats()
     srv.close()
     p50, p99 = np.percentile(lat2, [50, 99])
+    # latency by resolution: the small cells wait in the same queue as the
+    # large ones, so their tail shows how the stream is shared
+    by_res = {}
+    for i, q in enumerate(poi_reqs):
+        by_res.setdefault("x".join(map(str, q[0])), []).append(lat2[i])
+    per_res = {k: {"requests": len(v),
+                   "p50_ms": round(1e3 * float(np.percentile(v, 50)), 2),
+                   "p99_ms": round(1e3 * float(np.percentile(v, 99)), 2)}
+               for k, v in sorted(by_res.items(), key=lambda kv: (len(kv[0].split("x")),
+                                                                   int(kv[0].split("x")[0])))}
     out = {
         "metric": "RVE solves/s",
         "value": round(throughput, 2),
@@ -624,6 +634,7 @@ def c4_main(args):
         "poisson": {"rate_per_s": round(rate, 2), "load": args.load, "requests": n,
                     "wall_s": round(wall2, 3), "solves_per_s": round(n / wall2, 2),
                     "p50_ms": round(1e3 * p50, 2), "p99_ms": round(1e3 * p99, 2),
+                    "per_resolution": per_res,
                     "failed": sum(1 for r in res2.values() if not r["ok"])},
         "server": st,
     }

@@ -218,6 +218,32 @@ template <typename T, int N, int E, int DIM, int H>
 constexpr size_t lds_axis0() {
   return (size_t)N * lstride(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM) * sizeof(c_t<T>);
 }
+// Twiddle table W_N^m, m < N, in LDS: a workgroup copies the table after its
+// B bytes of lines, and the passes read it there (LDS reads, mostly
+// broadcasts) instead of through the vector-memory path, where every
+// butterfly's loads wait out the L1/L2 latency.  Only where the N extra
+// values leave the number of workgroups the LDS holds per CU unchanged.
+template <typename T, int N>
+struct TwLds {
+  static constexpr size_t tab = (size_t)N * sizeof(c_t<T>);
+  static constexpr size_t off(size_t B) { return (B + 15) / 16 * 16; }
+  static constexpr bool on(size_t B) { return 160 * 1024 / B == 160 * 1024 / (off(B) + tab); }
+  static constexpr size_t total(size_t B) { return on(B) ? off(B) + tab : B; }
+};
+// the table the passes read: the LDS copy where TwLds::on(B) (every thread
+// of the workgroup must call this), else tw itself
+template <typename T, int N, int NT, size_t B>
+__device__ __forceinline__ const c_t<T>* twiddles_for(const c_t<T>* __restrict__ tw,
+                                                      unsigned char* smem) {
+  if constexpr (TwLds<T, N>::on(B)) {
+    c_t<T>* t = reinterpret_cast<c_t<T>*>(smem + TwLds<T, N>::off(B));
+    for (int i = threadIdx.x; i < N; i += NT) t[i] = tw[i];
+    __syncthreads();
+    return t;
+  } else {
+    return tw;
+  }
+}
 // Waves per SIMD the register allocation has to leave room for: two
 // workgroups per CU wherever the 160 KiB of LDS hold two (one otherwise).
 // Without the bound the compiler may take enough registers per lane that a
@@ -256,7 +282,8 @@ MUGPU_WPE((s_lines<T, N, E>() * (N / E)), (lds_strided<T, N, E>())) void k_strid
     c_t<T>* __restrict__ data, int64_t inner, const c_t<T>* __restrict__ tw, const int* active) {
   if (active && !active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  strided_body<T, N, E, SG>(data, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+  const c_t<T>* t = twiddles_for<T, N, s_lines<T, N, E>() * (N / E), lds_strided<T, N, E>()>(tw, smem);
+  strided_body<T, N, E, SG>(data, inner, t, smem, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 // batched (FP32, one RVE per blockIdx.z)
@@ -267,7 +294,8 @@ MUGPU_WPE((s_lines<float, N, E>() * (N / E)), (lds_strided<float, N, E>())) void
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  strided_body<float, N, E, SG>(sl.spec, inner, tw, smem, blockIdx.x, blockIdx.y, gridDim.x,
+  const float2* t = twiddles_for<float, N, s_lines<float, N, E>() * (N / E), lds_strided<float, N, E>()>(tw, smem);
+  strided_body<float, N, E, SG>(sl.spec, inner, t, smem, blockIdx.x, blockIdx.y, gridDim.x,
                              gridDim.y);
 }
 
@@ -340,7 +368,10 @@ template <typename T, int N, int E, int DIM, int H>
 __global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM * (N / E)) void k_axis0_fast(XArgs<T> a) {
   if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  axis0_body<T, N, E, DIM, H>(a, smem, blockIdx.x, gridDim.x);
+  XArgs<T> b = a;
+  b.tw = twiddles_for<T, N, x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM * (N / E),
+                      lds_axis0<T, N, E, DIM, H>()>(a.tw, smem);
+  axis0_body<T, N, E, DIM, H>(b, smem, blockIdx.x, gridDim.x);
 }
 
 template <int N, int E, int DIM, int H>
@@ -355,6 +386,8 @@ __global__ __launch_bounds__(x_kpts(N, DIM, true, E, H) * DIM * DIM * (N / E)) v
   for (int d = 0; d < 3; ++d) a.L[d] = sl.L[d];
   a.norm = (float)sl.norm;
   extern __shared__ __align__(16) unsigned char smem[];
+  a.tw = twiddles_for<float, N, x_kpts(N, DIM, true, E, H) * DIM * DIM * (N / E),
+                      lds_axis0<float, N, E, DIM, H>()>(shape.tw, smem);
   axis0_body<float, N, E, DIM, H>(a, smem, blockIdx.x, gridDim.x);
 }
 
@@ -507,7 +540,9 @@ __global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
 MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM, H>())) void k_r2c_fast(LArgs<T> a) {
   if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  r2c_body<T, N, E, DIM, IN, ISO, H>(a, smem, blockIdx.x);
+  LArgs<T> b = a;
+  b.tw = twiddles_for<T, N, z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), lds_lines<T, N, DIM, H>()>(a.tw, smem);
+  r2c_body<T, N, E, DIM, IN, ISO, H>(b, smem, blockIdx.x);
 }
 
 // batched: the first iteration of a solve takes p as it is, later ones
@@ -528,8 +563,8 @@ MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM,
   a.tan_kind = sl.tan_kind;
   a.spec = sl.spec;
   a.rows = rows;
-  a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
+  a.tw = twiddles_for<float, N, z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), lds_lines<float, N, DIM, H>()>(tw, smem);
   if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN, true, H>(a, smem, blockIdx.x);
   else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE, true, H>(a, smem, blockIdx.x);
 }
@@ -639,7 +674,9 @@ __global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
 MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_c2r<T, N, E, DIM, H>())) void k_c2r_fast(LArgs<T> a) {
   if (a.active && !a.active[0]) return;
   extern __shared__ __align__(16) unsigned char smem[];
-  c2r_body<T, N, E, DIM, OUT, H>(a, smem, blockIdx.x);
+  LArgs<T> b = a;
+  b.tw = twiddles_for<T, N, z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), lds_c2r<T, N, E, DIM, H>()>(a.tw, smem);
+  c2r_body<T, N, E, DIM, OUT, H>(b, smem, blockIdx.x);
 }
 
 template <int N, int E, int DIM, int H>
@@ -654,8 +691,8 @@ MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_c2r<float, N, E, DIM
   a.dotv = sl.p;
   a.partials = sl.zpart;
   a.rows = rows;
-  a.tw = tw;
   extern __shared__ __align__(16) unsigned char smem[];
+  a.tw = twiddles_for<float, N, z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), lds_c2r<float, N, E, DIM, H>()>(tw, smem);
   c2r_body<float, N, E, DIM, OUT_DOT, H>(a, smem, blockIdx.x);
 }
 

@@ -22,7 +22,7 @@ int axis0_n(FftPlan& p, const ProjParams& pj, void* spec) {
   a.norm = (T)pj.norm;
   a.active = p.active;
   constexpr int KP = x_kpts(N, DIM, sizeof(T) == 4, E, H), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  constexpr size_t lds = lds_axis0<T, N, E, DIM, H>();
+  constexpr size_t lds = TwLds<T, N>::total(lds_axis0<T, N, E, DIM, H>());
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP));
   hipLaunchKernelGGL((k_axis0_fast<T, N, E, DIM, H>), grid, dim3(NT), lds, stream(), a);
   MUGPU_HIP(hipGetLastError());

@@ -11,7 +11,7 @@ int b_r2c(FftPlan& p, const BatchSlot* slots, int ns) {
   constexpr int RB = z_rows(N, DIM, H);
   constexpr int NT = RB / 2 * DIM * DIM * (N / E);
   const int64_t rows = p.N / N;
-  constexpr size_t lds = lds_lines<float, N, DIM, H>();
+  constexpr size_t lds = TwLds<float, N>::total(lds_lines<float, N, DIM, H>());
   const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_r2c_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
@@ -24,7 +24,7 @@ int b_c2r(FftPlan& p, const BatchSlot* slots, int ns) {
   constexpr int RB = z_rows(N, DIM, H), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
   const int64_t rows = p.N / N;
-  constexpr size_t lds = lds_c2r<float, N, E, DIM, H>();
+  constexpr size_t lds = TwLds<float, N>::total(lds_c2r<float, N, E, DIM, H>());
   const dim3 grid((unsigned)((rows + RB - 1) / RB), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_c2r_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, rows,
                      tw_of<float>(p, DIM - 1));
@@ -40,7 +40,7 @@ int b_strided(FftPlan& p, int ax, int sign, const BatchSlot* slots, int ns) {
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   constexpr int L = s_lines<float, N, E>();
   constexpr int NT = L * (N / E);
-  constexpr size_t lds = lds_strided<float, N, E>();
+  constexpr size_t lds = TwLds<float, N>::total(lds_strided<float, N, E>());
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer, (unsigned)ns);
   if (sign < 0)
     hipLaunchKernelGGL((k_strided_batch<N, E, -1>), grid, dim3(NT), lds, stream(), slots, inner,
@@ -60,7 +60,7 @@ int b_axis0(FftPlan& p, const BatchSlot* slots, int ns) {
   for (int d = 0; d < 3; ++d) a.res[d] = p.res[d];
   a.hlast = p.hres[DIM - 1];
   constexpr int KP = x_kpts(N, DIM, true, E, H), NC = DIM * DIM, L = KP * NC, NT = L * (N / E);
-  constexpr size_t lds = lds_axis0<float, N, E, DIM, H>();
+  constexpr size_t lds = TwLds<float, N>::total(lds_axis0<float, N, E, DIM, H>());
   const dim3 grid((unsigned)((a.plane + KP - 1) / KP), 1, (unsigned)ns);
   hipLaunchKernelGGL((k_axis0_batch<N, E, DIM, H>), grid, dim3(NT), lds, stream(), slots, a);
   MUGPU_HIP(hipGetLastError());

@@ -18,7 +18,7 @@ int c2r_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.active = p.active;
   constexpr int RB = z_rows(N, DIM, H), NC = DIM * DIM;
   constexpr int NT = RB / 2 * NC * (N / E);
-  constexpr size_t lds = lds_c2r<T, N, E, DIM, H>();
+  constexpr size_t lds = TwLds<T, N>::total(lds_c2r<T, N, E, DIM, H>());
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   if (op.out_mode == OUT_DOT)
     hipLaunchKernelGGL((k_c2r_fast<T, N, E, DIM, OUT_DOT, H>), grid, dim3(NT), lds, stream(), a);

@@ -22,7 +22,7 @@ int r2c_n(FftPlan& p, const OpSpec& op, void* spec) {
   a.x = reinterpret_cast<T*>(op.xvec);
   constexpr int RB = z_rows(N, DIM, H);
   constexpr int NT = RB / 2 * DIM * DIM * (N / E);
-  constexpr size_t lds = lds_lines<T, N, DIM, H>();
+  constexpr size_t lds = TwLds<T, N>::total(lds_lines<T, N, DIM, H>());
   const dim3 grid((unsigned)((a.rows + RB - 1) / RB));
   const bool iso = op.tan_kind == TAN_ISO;
   // the fused x update goes with the p update only

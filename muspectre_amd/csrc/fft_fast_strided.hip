@@ -14,7 +14,7 @@ int strided_n(FftPlan& p, int ax, void* spec, int sign) {
   for (int d = 0; d < ax; ++d) outer *= p.hres[d];
   constexpr int L = s_lines<T, N, E>();
   constexpr int NT = L * (N / E);
-  constexpr size_t lds = lds_strided<T, N, E>();
+  constexpr size_t lds = TwLds<T, N>::total(lds_strided<T, N, E>());
   const dim3 grid((unsigned)((inner + L - 1) / L), (unsigned)outer);
   c_t<T>* data = reinterpret_cast<c_t<T>*>(spec);
   if (sign < 0)
