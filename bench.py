@@ -69,6 +69,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="c4: launch the batched iterations directly instead of replaying "
+                         "captured graphs (the same kernels; for kernel tracers)")
     return ap.parse_args()
 
 
@@ -586,7 +589,8 @@ def c4_main(args):
     print("bench: c4 generating {} + {} requests".format(n, n), file=sys.stderr, flush=True)
     sat_reqs = c4_stream(n, 1)
     poi_reqs = c4_stream(n, 100001)
-    srv = serving.RVEServer(max_batch=32, iters_per_launch=8, cg_tol=1e-6, cg_maxiter=4000)
+    srv = serving.RVEServer(max_batch=32, iters_per_launch=8, cg_tol=1e-6, cg_maxiter=4000,
+                            use_graphs=not args.no_graphs)
     # warm-up: one request of each resolution (plans, graphs)
     warm = {}
     for q in sat_reqs + poi_reqs:
@@ -628,7 +632,8 @@ def c4_main(args):
         "dtype": "f32-operator/f64-refined",
         "data": "synthetic",
         "config": {"workload": "c4", "requests": n, "mix": "3D p=.7 {32,48,64,96}^3, 2D {64..512}^2",
-                   "cg_tol": 1e-6, "formulation": "small", "max_batch": 32, "iters_per_launch": 8},
+                   "cg_tol": 1e-6, "formulation": "small", "max_batch": 32, "iters_per_launch": 8,
+                   "graphs": not args.no_graphs},
         "saturated": {"requests": n, "wall_s": round(wall, 3), "solves_per_s": round(throughput, 2),
                       "voxel_cg_iters_per_s": round(voxel_iters / wall, 1), "failed": failed},
         "poisson": {"rate_per_s": round(rate, 2), "load": args.load, "requests": n,

@@ -241,8 +241,15 @@ struct Request {
   }
 };
 
+// One resolution class: its plan, its batch slots and its two streams (the
+// batched inner iterations on sb, the requests' own stages on sr), so that
+// the work of one resolution never queues behind another's.  A request's
+// stages and its class's chunks hand over to each other only after the host
+// has seen the previous one complete (hipEventQuery), which orders them
+// across the two streams.
 struct ResClass {
   int dim = 0, res[3] = {1, 1, 1};
+  hipStream_t sb = nullptr, sr = nullptr;
   FftPlan plan;
   int64_t N = 0;
   int* d_iota = nullptr;  // pixel list 0 .. N-1 of the material kernel
@@ -310,7 +317,6 @@ class EventPool {
 
 struct mugpu_server_s {
   mugpu_server_config cfg{};
-  hipStream_t sb = nullptr, sr = nullptr;  // batch stream, request stream
   Arena arena;
   PinnedPool pinned;
   EventPool events;
@@ -336,6 +342,7 @@ struct mugpu_server_s {
   int process_chunk(ResClass* c);
   int graph_for(ResClass* c, int cap, hipGraphExec_t* out);
   int record(Request* r) {
+    hipStream_t sr = r->cls->sr;
     MUGPU_HIP(hipMemcpyAsync(r->h, r->scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, sr));
     MUGPU_HIP(hipEventRecord(r->ev, sr));
     return MUGPU_OK;
@@ -348,6 +355,8 @@ mugpu_server_s::~mugpu_server_s() {
     ResClass* c = kv.second.get();
     for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
     if (c->ev) (void)hipEventDestroy(c->ev);
+    if (c->sb) (void)hipStreamDestroy(c->sb);
+    if (c->sr) (void)hipStreamDestroy(c->sr);
     if (c->d_iota) (void)hipFree(c->d_iota);
     if (c->d_slots) (void)hipFree(c->d_slots);
     if (c->d_ctl) (void)hipFree(c->d_ctl);
@@ -356,8 +365,6 @@ mugpu_server_s::~mugpu_server_s() {
     if (c->h_ctl_down) (void)hipHostFree(c->h_ctl_down);
     plan_free(c->plan);
   }
-  if (sb) (void)hipStreamDestroy(sb);
-  if (sr) (void)hipStreamDestroy(sr);
 }
 
 int mugpu_server_s::class_of(Request* r) {
@@ -390,6 +397,8 @@ int mugpu_server_s::class_of(Request* r) {
   MUGPU_HIP(hipMemcpy(c->d_slots, c->h_slots, sizeof(BatchSlot) * cap, hipMemcpyHostToDevice));
   MUGPU_HIP(hipMemcpy(c->d_ctl, c->h_ctl_up, sizeof(int) * CTL_COUNT * cap, hipMemcpyHostToDevice));
   MUGPU_HIP(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
+  MUGPU_HIP(hipStreamCreateWithFlags(&c->sb, hipStreamNonBlocking));
+  MUGPU_HIP(hipStreamCreateWithFlags(&c->sr, hipStreamNonBlocking));
   r->cls = c.get();
   classes.emplace(key, std::move(c));
   return MUGPU_OK;
@@ -428,6 +437,7 @@ int mugpu_server_s::admit() {
       return MUGPU_EMEMORY;
     }
     MUGPU_TRY(class_of(r));
+    hipStream_t sr = r->cls->sr;
     running.push_back(r);
     ++stats[1];
     stats[8] = std::max<int64_t>(stats[8], (int64_t)running.size());
@@ -454,6 +464,7 @@ int mugpu_server_s::admit() {
 // one Newton iteration begins: stress and tangent, rhs = -G:P, |rhs|
 // (solvers.cc:165-176)
 int mugpu_server_s::start_eval(Request* r) {
+  hipStream_t sr = r->cls->sr;
   const int newton_bound = r->maxiter;
   if (r->newt_iter >= newton_bound || r->has_converged) return finish(r);
   set_current_stream(sr);
@@ -485,6 +496,7 @@ int mugpu_server_s::start_eval(Request* r) {
 
 // r64 = b - A x in FP64, |r64|^2 (the refinement step of the mixed solve)
 int mugpu_server_s::start_resid(Request* r) {
+  hipStream_t sr = r->cls->sr;
   set_current_stream(sr);
   OpSpec op;
   op.in_mode = IN_TAN;
@@ -506,12 +518,14 @@ int mugpu_server_s::start_resid(Request* r) {
 }
 
 int mugpu_server_s::after_inner(Request* r) {
+  hipStream_t sr = r->cls->sr;
   set_current_stream(sr);
   MUGPU_TRY(add_scaled_convert(r->x, r->x32, 1.0 / r->s, r->nd));
   return start_resid(r);
 }
 
 int mugpu_server_s::finish(Request* r) {
+  hipStream_t sr = r->cls->sr;
   set_current_stream(sr);
   r->converged = r->test(cfg.newton_tol, cfg.equil_tol);
   const int g = (int)std::min<int64_t>(kColBlocks, (r->N + kColT - 1) / kColT);
@@ -534,6 +548,7 @@ int mugpu_server_s::finish(Request* r) {
 // the refinement step with the residual |r64|^2 = r->rr known (the loop of
 // the mixed-precision solve: stop, or start the next inner CG)
 int mugpu_server_s::residual_known(Request* r) {
+  hipStream_t sr = r->cls->sr;
   ++r->refs;
   if (!(r->rr > r->tol2) || r->its >= r->maxiter || !(r->rr == r->rr)) {
     r->counter += r->its;
@@ -563,6 +578,7 @@ int mugpu_server_s::residual_known(Request* r) {
 
 // the device work of r's current stage has completed: take the next step
 int mugpu_server_s::advance(Request* r) {
+  hipStream_t sr = r->cls->sr;
   const double* h = r->h;
   const double tol = cfg.cg_tol;
   switch (r->st) {
@@ -625,6 +641,7 @@ int mugpu_server_s::advance(Request* r) {
 }
 
 int mugpu_server_s::graph_for(ResClass* c, int cap, hipGraphExec_t* out) {
+  hipStream_t sb = c->sb;
   auto it = c->graphs.find(cap);
   if (it != c->graphs.end()) {
     *out = it->second;
@@ -655,6 +672,7 @@ int mugpu_server_s::graph_for(ResClass* c, int cap, hipGraphExec_t* out) {
 
 // pack the running slots, fill free ones from the queue, launch K iterations
 int mugpu_server_s::launch_chunk(ResClass* c) {
+  hipStream_t sb = c->sb;
   std::vector<Request*> act(c->slot_req.begin(), c->slot_req.end());
   while ((int)act.size() < cfg.max_batch && !c->waiting.empty()) {
     Request* r = c->waiting.front();
@@ -773,8 +791,6 @@ extern "C" int mugpu_server_create(const mugpu_server_config* cfg, mugpu_server*
   }
   MUGPU_TRY(s->arena.init(budget));
   s->stats[7] = (int64_t)budget;
-  MUGPU_HIP(hipStreamCreateWithFlags(&s->sb, hipStreamNonBlocking));
-  MUGPU_HIP(hipStreamCreateWithFlags(&s->sr, hipStreamNonBlocking));
   *out = s.release();
   return MUGPU_OK;
 }
