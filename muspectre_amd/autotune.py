@@ -8,7 +8,9 @@ variants for the power-of-two lengths: 16 complex values per thread, or 8
 with twice the threads on the same tile (bit 0 of the variant).  The
 last-axis passes and the first-axis projection pass also come with half the
 rows or k-points per workgroup (bit 1): half the LDS, more workgroups per CU
-(and, on the first axis, shorter HBM runs).  Which one
+(and, on the first axis, shorter HBM runs).  The first-axis projection pass
+also exists with an occupancy hint on its register allocation (bit 2),
+which changes its instruction schedule.  Which one
 is fastest depends on the pass, the length, the dimension and the
 precision.  This script times every variant of every pass on the GPU
 (mugpu_fft_tune_pass, zeroed scratch buffers of a grid large enough to fill
@@ -74,7 +76,8 @@ def main():
                     for n in LENGTHS:
                         res = grid(pass_, n, dim)
                         t = {}
-                        for v in ((0, 1, 2, 3) if pass_ in (0, 2, 3) else (0, 1)):
+                        for v in (range(8) if pass_ == 2 else (0, 1, 2, 3) if pass_ in (0, 3)
+                                  else (0, 1)):
                             try:
                                 t[v] = time_pass(lib, res, prec, pass_, v, iso, args.reps)
                             except _capi.MugpuError:
@@ -87,7 +90,7 @@ def main():
                         rows.append((dim, pname, PASSES[pass_], iso, n, res, t, best))
                         print("dim {} {} {:26s} iso {} n {:4d} grid {}: {} ms -> {}".format(
                             dim, pname, PASSES[pass_], iso, n, res,
-                            " / ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(4)),
+                            " / ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(8)),
                             best), flush=True)
                         if best:
                             entries.append((pass_, n, dim, prec, iso, best))
@@ -113,15 +116,16 @@ def main():
                         args.reps, 100 * args.margin, 1000 * args.min_gain_ms))
             f.write("Variants: 0 = 16 (12) values per thread, 1 = 8 values per thread, "
                     "2 and 3 = the same with half the rows (last-axis passes) or k-points "
-                    "(first-axis projection pass) per workgroup; - = no such variant at this "
-                    "length.\n\n")
+                    "(first-axis projection pass) per workgroup, 4-7 = variants 0-3 of the "
+                    "first-axis projection pass with the occupancy hint; - = no such variant "
+                    "at this length.\n\n")
             f.write("| dim | precision | pass | isotropic tangent | n | grid | ms, variant 0 | "
-                    "variant 1 | variant 2 | variant 3 | chosen |\n"
-                    "|---|---|---|---|---|---|---|---|---|---|---|\n")
+                    "1 | 2 | 3 | 4 | 5 | 6 | 7 | chosen |\n"
+                    "|---|---|---|---|---|---|---|---|---|---|---|---|---|---|---|\n")
             for dim, pname, pn, iso, n, res, t, best in rows:
                 f.write("| {} | {} | {} | {} | {} | {} | {} | {} |\n".format(
                     dim, pname, pn, iso, n, "x".join(map(str, res)),
-                    " | ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(4)), best))
+                    " | ".join("{:.3f}".format(t[v]) if v in t else "-" for v in range(8)), best))
     print("wrote {} ({} entries)".format(os.path.abspath(args.out), len(entries)))
 
 

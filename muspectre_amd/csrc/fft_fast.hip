@@ -74,7 +74,8 @@ int fast_variant(int pass, int n, int dim, int prec, int iso) {
 
 bool fast_variant_valid(int pass, int n, int dim, int prec, int variant) {
   const bool halves = pass == FV_AXIS0 || pass == FV_R2C || pass == FV_C2R;
-  if (variant < 0 || variant > (halves ? 3 : 1)) return false;
+  // bit 2: the first-axis kernel with the occupancy hint (k_axis0_bounded)
+  if (variant < 0 || variant > (pass == FV_AXIS0 ? 7 : halves ? 3 : 1)) return false;
   if ((variant & 1) && n % 3 == 0) return false;
   if (variant & 2) {
     if (pass != FV_AXIS0) return z_rows(n, dim, 1) != z_rows(n, dim, 0);

@@ -105,6 +105,8 @@ int batch_operator_dim(FftPlan& p, const BatchSlot* slots, int ns) {
     if (rc) return rc;
   }
   timing_begin(MUGPU_K_AXIS0);
+// (bit 2 of the first-axis variant, the occupancy hint, belongs to the
+// single-cell kernel; the batched one takes bits 0 and 1)
 #define C_(n, e) b_axis0_h<n, e, DIM>(p, slots, ns, (fv[FV_AXIS0][0] >> 1) & 1)
   rc = [&]() -> int { MUGPU_FAST_SWITCH(p.res[0], fv[FV_AXIS0][0] & 1, C_) return MUGPU_EARG; }();
 #undef C_
