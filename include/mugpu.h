@@ -91,6 +91,13 @@ int mugpu_device_info(char *name, int name_len, int *cu_count,
  * counterpart) returns the cached blocks to the device after the library
  * stream's work; an allocation that fails does so by itself and retries. */
 int mugpu_alloc(size_t bytes, void **d_ptr);
+/* Test hook for the failure paths of the library's own device allocations
+ * (no reference counterpart; for tests of out-of-memory handling): from now
+ * on the allocations of cells, projections, the mixed-precision work vectors
+ * and the bandwidth probes (not mugpu_alloc) succeed `after` more times, and
+ * every later one fails with MUGPU_EMEMORY as if the device were full, until
+ * the hook is called with after < 0 (the default: no injected failure). */
+int mugpu_debug_fail_allocations(int after);
 int mugpu_free(void *d_ptr);
 int mugpu_release_cached(void);
 int mugpu_upload(void *d_dst, const void *h_src, size_t bytes);

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -420,8 +421,21 @@ void release_cached_locked() {
 }
 }  // namespace
 
+// allocations dev_alloc still grants before it reports the device full
+// (-1: no limit), set by mugpu_debug_fail_allocations for the tests of the
+// failure paths
+static std::atomic<int> g_fail_alloc_after{-1};
+
 int mugpu::dev_alloc(void** p, size_t bytes, const char* what) {
   *p = nullptr;
+  for (int left = g_fail_alloc_after.load(); left >= 0;) {
+    if (left == 0) {
+      set_error("device allocation of %s (%zu bytes) refused: a failure injected by "
+                "mugpu_debug_fail_allocations", what, bytes);
+      return MUGPU_EMEMORY;
+    }
+    if (g_fail_alloc_after.compare_exchange_weak(left, left - 1)) break;
+  }
   hipError_t e = hipMalloc(p, bytes ? bytes : 16);
   if (e != hipSuccess) {
     (void)hipGetLastError();
@@ -441,6 +455,11 @@ int mugpu::dev_alloc(void** p, size_t bytes, const char* what) {
     set_error("device allocation of %s (%zu bytes) failed: %s", what, bytes, hipGetErrorString(e));
     return MUGPU_EMEMORY;
   }
+  return MUGPU_OK;
+}
+
+extern "C" int mugpu_debug_fail_allocations(int after) {
+  g_fail_alloc_after.store(after < 0 ? -1 : after);
   return MUGPU_OK;
 }
 
