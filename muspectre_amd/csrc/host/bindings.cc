@@ -659,6 +659,19 @@ PYBIND11_MODULE(_muSpectre, mod) {
       .value("measure", FFT_PlanFlags::measure)
       .value("patient", FFT_PlanFlags::patient);
   py::enum_<Precision>(mod, "Precision").value("fp64", Precision::fp64).value("fp32", Precision::fp32);
+  // the notice executables print at start-up (common.hh:130-134, bound by
+  // bind_py_common.cc:158 with these argument names)
+  mod.def(
+      "banner",
+      [](const std::string& name, Uint year, const std::string& holder) {
+        py::print("");
+        py::print("µSpectre " + name);
+        py::print("Copyright © " + std::to_string(year) + "  " + holder);
+        py::print("Free software without any warranty; see the licence file for the terms "
+                  "of use and redistribution.");
+        py::print("");
+      },
+      "name"_a, "year"_a, "copyright_holder"_a);
   bind_common_helpers<2>(mod);
   bind_common_helpers<3>(mod);
 

@@ -74,3 +74,13 @@ def test_le3_le4_pixels_need_moduli(µ):
         mat = getattr(µ.material, name)("x")
         with pytest.raises(TypeError):
             mat.add_pixel([0, 0])
+
+
+def test_banner_prints_name_and_copyright(capsys):
+    """bind_py_common.cc:158 binds common.hh's banner(name, year,
+    copyright_holder): the notice names the program and the holder."""
+    from muspectre_amd import _muSpectre
+    _muSpectre.banner("demo", 2026, "someone")
+    out = capsys.readouterr().out
+    assert "µSpectre demo" in out and "Copyright © 2026  someone" in out
+    _muSpectre.banner(name="demo", year=2026, copyright_holder="someone")
