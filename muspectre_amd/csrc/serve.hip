@@ -852,6 +852,12 @@ extern "C" int mugpu_server_submit(mugpu_server s, int dim, const int* res, cons
 
 extern "C" int mugpu_server_progress(mugpu_server s, int wait, int* unfinished) {
   if (!s) return MUGPU_EARG;
+  // the stages queue their work on the streams of their resolution class; on
+  // every return, an error's included, the library's own stream is current
+  // again, so that later calls outside the server do not land on them
+  struct StreamReset {
+    ~StreamReset() { set_current_stream(nullptr); }
+  } reset;
   MUGPU_TRY(s->admit());
   bool moved = false;
   // requests whose stage completed
@@ -883,7 +889,6 @@ extern "C" int mugpu_server_progress(mugpu_server s, int wait, int* unfinished) 
                                   [](Request* r) { return r->st == ST_DONE; }),
                    s->running.end());
   MUGPU_TRY(s->admit());
-  set_current_stream(nullptr);
   if (wait && !moved) {
     // block on the oldest outstanding device work
     for (Request* r : s->running)
