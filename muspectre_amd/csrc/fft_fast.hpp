@@ -380,10 +380,10 @@ __global__ __launch_bounds__(x_kpts(N, DIM, sizeof(T) == 4, E, H) * DIM * DIM * 
 }
 
 // The same pass with the register allocation told to leave room for
-// axis0_hint<T>() waves per SIMD (variant bit 2).  Where that bound is not
-// below the unbounded kernel's occupancy it changes the allocation and the
-// instruction schedule rather than the occupancy; at 512 (E = 16) it
-// shortens the FP64 pass by 17 % and the FP32 one by 3 %, with the same
+// axis0_hint<T>() waves per SIMD (variant bit 2).  Where the LDS already
+// caps the workgroups per CU, as at 512, the bound changes the register
+// allocation and the instruction schedule, not the occupancy; there (E = 16)
+// it shortens the FP64 pass by 17 % and the FP32 one by 3 %, with the same
 // results bit for bit.  The tuned table chooses it per length.
 template <typename T>
 __host__ __device__ constexpr int axis0_hint() {
