@@ -18,8 +18,8 @@ def pytest_configure(config):
 def oracle():
     from oracle import oracle as O
     O.lib()
-    # the oracle's OpenMP loops on at most 16 host threads (the CPU share of
-    # one GPU box); only the summation order of its dot products depends on it
+    # the oracle's OpenMP loops on at most 16 host threads; only the summation
+    # order of its dot products depends on the count
     O.set_threads(min(16, os.cpu_count() or 1))
     return O
 
