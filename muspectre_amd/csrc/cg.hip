@@ -292,21 +292,21 @@ __global__ __launch_bounds__(1024) void k_finalize_batch(const BatchSlot* __rest
   }
 }
 
+// r += alpha Ap and the partials of r.r per slot; the x update rides on the
+// next iteration's r2c (k_r2c_batch), and the last one is folded into the
+// refinement's correction of x (add_scaled_convert with p)
 __global__ __launch_bounds__(kRT) void k_cg_update_batch(const BatchSlot* __restrict__ slots,
                                                          int64_t n) {
   const BatchSlot& sl = slots[blockIdx.z];
   if (!sl.ctl[CTL_ACTIVE]) return;
   __shared__ double red[kRT];
-  float* __restrict__ x = sl.x;
   float* __restrict__ r = sl.r;
-  const float* __restrict__ p = sl.p;
   const float* __restrict__ Ap = sl.Ap;
   const double alpha = sl.scal[SC_ALPHA];
   const float al = (float)alpha;
   double acc = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kRT;
   for (int64_t i = (int64_t)blockIdx.x * kRT + threadIdx.x; i < n; i += stride) {
-    x[i] += al * p[i];
     const float rv = r[i] + al * Ap[i];
     r[i] = rv;
     acc += (double)rv * (double)rv;

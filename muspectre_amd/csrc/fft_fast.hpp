@@ -568,8 +568,9 @@ MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<T, N, DIM, H>(
 }
 
 // batched: the first iteration of a solve takes p as it is, later ones
-// update it (p = -r + beta p) on the way in; the served material is
-// isotropic (TAN_ISO tangents)
+// update it (p = -r + beta p) on the way in and apply the previous
+// iteration's x update (x += alpha p_old), as the single-cell solve does;
+// the served material is isotropic (TAN_ISO tangents)
 template <int N, int E, int DIM, int H>
 __global__ __launch_bounds__(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E))
 MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM, H>())) void k_r2c_batch(
@@ -585,10 +586,11 @@ MUGPU_WPE(z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), (lds_lines<float, N, DIM,
   a.tan_kind = sl.tan_kind;
   a.spec = sl.spec;
   a.rows = rows;
+  a.x = sl.x;
   extern __shared__ __align__(16) unsigned char smem[];
   a.tw = twiddles_for<float, N, z_rows(N, DIM, H) / 2 * DIM * DIM * (N / E), lds_lines<float, N, DIM, H>()>(tw, smem);
   if (sl.ctl[CTL_ITER] == 0) r2c_body<float, N, E, DIM, IN_TAN, true, H>(a, smem, blockIdx.x);
-  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE, true, H>(a, smem, blockIdx.x);
+  else r2c_body<float, N, E, DIM, IN_TAN_PUPDATE_X, true, H>(a, smem, blockIdx.x);
 }
 
 // ---- last axis: c2r, output stage ---------------------------------------------

@@ -336,7 +336,7 @@ struct mugpu_server_s {
   int start_eval(Request* r);
   int start_resid(Request* r);
   int finish(Request* r);
-  int after_inner(Request* r);
+  int after_inner(Request* r, bool ran);
   int residual_known(Request* r);
   int launch_chunk(ResClass* c);
   int process_chunk(ResClass* c);
@@ -517,10 +517,14 @@ int mugpu_server_s::start_resid(Request* r) {
   return record(r);
 }
 
-int mugpu_server_s::after_inner(Request* r) {
+// x += |r64| d after an inner solve; when it ran in the batch, the x update
+// of its last iteration (alpha and p of that iteration, left pending by the
+// batched r2c) is folded in, as in the single-cell mixed solve
+int mugpu_server_s::after_inner(Request* r, bool ran) {
   hipStream_t sr = r->cls->sr;
   set_current_stream(sr);
-  MUGPU_TRY(add_scaled_convert(r->x, r->x32, 1.0 / r->s, r->nd));
+  MUGPU_TRY(add_scaled_convert(r->x, r->x32, 1.0 / r->s, r->nd, ran ? r->p32 : nullptr,
+                               ran ? r->scal : nullptr));
   return start_resid(r);
 }
 
@@ -610,7 +614,7 @@ int mugpu_server_s::advance(Request* r) {
     case ST_WAIT_RDR: {
       const double rdr = h[SC_RDR];
       const double tol2i = cfg.inner_tol * cfg.inner_tol * rdr;
-      if (r->its >= r->maxiter) return after_inner(r);  // no inner iteration
+      if (r->its >= r->maxiter) return after_inner(r, false);  // no inner iteration
       r->h[kHostTol2] = tol2i;
       r->inner_maxit = (int)(r->maxiter - r->its);
       r->st = ST_BATCH_QUEUE;
@@ -751,7 +755,7 @@ int mugpu_server_s::process_chunk(ResClass* c) {
     }
     // inner CG done: its iterations join the solve's count
     r->its += r->inner_iter;
-    MUGPU_TRY(after_inner(r));
+    MUGPU_TRY(after_inner(r, true));
   }
   c->slot_req = keep;
   return MUGPU_OK;
