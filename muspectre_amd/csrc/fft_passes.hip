@@ -471,8 +471,8 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     std::vector<double> tw;
     twiddles(res[d], tw);
     std::vector<float> tw32(tw.begin(), tw.end());
-    MUGPU_HIP(hipMalloc(&p.tw64[d], tw.size() * sizeof(double)));
-    MUGPU_HIP(hipMalloc(&p.tw32[d], tw32.size() * sizeof(float)));
+    MUGPU_TRY(dev_alloc(&p.tw64[d], tw.size() * sizeof(double), "the twiddle tables"));
+    MUGPU_TRY(dev_alloc(&p.tw32[d], tw32.size() * sizeof(float), "the twiddle tables"));
     MUGPU_HIP(hipMemcpy(p.tw64[d], tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice));
     MUGPU_HIP(hipMemcpy(p.tw32[d], tw32.data(), tw32.size() * sizeof(float), hipMemcpyHostToDevice));
   }
@@ -568,7 +568,8 @@ int plan_init(FftPlan& p, int dim, const int* res, int nc, int prec) {
     const int rbmin = fast_rows_per_block(res[dim - 1], dim, 2);
     nzmax = (p.N / res[dim - 1] + rbmin - 1) / rbmin;
   }
-  MUGPU_HIP(hipMalloc(&p.partials, sizeof(double) * (size_t)(nzmax > kRedBlocks ? nzmax : kRedBlocks)));
+  MUGPU_TRY(dev_alloc(&p.partials, sizeof(double) * (size_t)(nzmax > kRedBlocks ? nzmax : kRedBlocks),
+                      "the transform's partial sums"));
   return MUGPU_OK;
 }
 

@@ -8,9 +8,10 @@
 // control flow of both is replayed step for step as an event-driven state
 // machine, so the host never blocks on one request while others could run.
 //
-// Device work of a request runs on the request stream, except its inner CG
-// iterations: those of all running requests of one resolution share batched
-// launches on the batch stream (one workgroup grid per launch, one RVE per
+// Each resolution class has two streams of its own.  Device work of a request
+// runs on its class's request stream, except its inner CG iterations: those
+// of all running requests of one resolution share batched launches on the
+// class's batch stream (one workgroup grid per launch, one RVE per
 // blockIdx.z; slots packed at the front).  A batch chunk is K iterations,
 // replayed from a captured hipGraph (one graph per power-of-two slot
 // count).  Every slot carries its own scalars and stopping test, evaluated
@@ -246,8 +247,26 @@ struct Request {
 // the work of one resolution never queues behind another's.  A request's
 // stages and its class's chunks hand over to each other only after the host
 // has seen the previous one complete (hipEventQuery), which orders them
-// across the two streams.
+// across the two streams.  It owns what it holds: a class that failed half
+// way through its set-up frees what it had taken; the server destroys its
+// classes only after the device is idle.
 struct ResClass {
+  ResClass() = default;
+  ResClass(const ResClass&) = delete;
+  ResClass& operator=(const ResClass&) = delete;
+  ~ResClass() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    if (ev) (void)hipEventDestroy(ev);
+    if (sb) (void)hipStreamDestroy(sb);
+    if (sr) (void)hipStreamDestroy(sr);
+    if (d_iota) (void)hipFree(d_iota);
+    if (d_slots) (void)hipFree(d_slots);
+    if (d_ctl) (void)hipFree(d_ctl);
+    if (h_slots) (void)hipHostFree(h_slots);
+    if (h_ctl_up) (void)hipHostFree(h_ctl_up);
+    if (h_ctl_down) (void)hipHostFree(h_ctl_down);
+    plan_free(plan);
+  }
   int dim = 0, res[3] = {1, 1, 1};
   hipStream_t sb = nullptr, sr = nullptr;
   FftPlan plan;
@@ -351,20 +370,7 @@ struct mugpu_server_s {
 
 mugpu_server_s::~mugpu_server_s() {
   (void)hipDeviceSynchronize();
-  for (auto& kv : classes) {
-    ResClass* c = kv.second.get();
-    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
-    if (c->ev) (void)hipEventDestroy(c->ev);
-    if (c->sb) (void)hipStreamDestroy(c->sb);
-    if (c->sr) (void)hipStreamDestroy(c->sr);
-    if (c->d_iota) (void)hipFree(c->d_iota);
-    if (c->d_slots) (void)hipFree(c->d_slots);
-    if (c->d_ctl) (void)hipFree(c->d_ctl);
-    if (c->h_slots) (void)hipHostFree(c->h_slots);
-    if (c->h_ctl_up) (void)hipHostFree(c->h_ctl_up);
-    if (c->h_ctl_down) (void)hipHostFree(c->h_ctl_down);
-    plan_free(c->plan);
-  }
+  classes.clear();
 }
 
 int mugpu_server_s::class_of(Request* r) {
@@ -382,10 +388,10 @@ int mugpu_server_s::class_of(Request* r) {
   const int cap = cfg.max_batch;
   std::vector<int> iota((size_t)c->N);
   for (int64_t i = 0; i < c->N; ++i) iota[(size_t)i] = (int)i;
-  MUGPU_HIP(hipMalloc((void**)&c->d_iota, sizeof(int) * (size_t)c->N));
+  MUGPU_TRY(dev_alloc(&c->d_iota, sizeof(int) * (size_t)c->N, "a resolution class's pixel list"));
   MUGPU_HIP(hipMemcpy(c->d_iota, iota.data(), sizeof(int) * (size_t)c->N, hipMemcpyHostToDevice));
-  MUGPU_HIP(hipMalloc((void**)&c->d_slots, sizeof(BatchSlot) * cap));
-  MUGPU_HIP(hipMalloc((void**)&c->d_ctl, sizeof(int) * CTL_COUNT * cap));
+  MUGPU_TRY(dev_alloc(&c->d_slots, sizeof(BatchSlot) * cap, "the batch slots"));
+  MUGPU_TRY(dev_alloc(&c->d_ctl, sizeof(int) * CTL_COUNT * cap, "the batch control words"));
   MUGPU_HIP(hipHostMalloc((void**)&c->h_slots, sizeof(BatchSlot) * cap, hipHostMallocDefault));
   MUGPU_HIP(hipHostMalloc((void**)&c->h_ctl_up, sizeof(int) * CTL_COUNT * cap, hipHostMallocDefault));
   MUGPU_HIP(hipHostMalloc((void**)&c->h_ctl_down, sizeof(int) * CTL_COUNT * cap, hipHostMallocDefault));
@@ -410,7 +416,27 @@ int mugpu_server_s::admit() {
     Request* r = queue.front();
     size_t off = 0;
     if (!arena.alloc(r->lay.total, &off)) break;
+    // its pinned words, its event and its resolution class; when one of them
+    // cannot be had, the request stays at the head of the queue holding
+    // nothing, and the error is returned
+    double* h = pinned.get();
+    hipEvent_t ev = h ? events.get() : nullptr;
+    int rc = MUGPU_OK;
+    if (!h || !ev) {
+      set_error("pinned host memory or events exhausted");
+      rc = MUGPU_EMEMORY;
+    } else {
+      rc = class_of(r);
+    }
+    if (rc != MUGPU_OK) {
+      if (h) pinned.put(h);
+      if (ev) events.put(ev);
+      arena.release(off, r->lay.total);
+      return rc;
+    }
     queue.pop_front();
+    r->h = h;
+    r->ev = ev;
     r->off = off;
     char* b = arena.base() + off;
     r->F = (double*)(b + r->lay.F);
@@ -430,13 +456,6 @@ int mugpu_server_s::admit() {
     r->part = (double*)(b + r->lay.part);
     r->zpart = (double*)(b + r->lay.zpart);
     r->colpart = (double*)(b + r->lay.colpart);
-    r->h = pinned.get();
-    r->ev = events.get();
-    if (!r->h || !r->ev) {
-      set_error("pinned host memory or events exhausted");
-      return MUGPU_EMEMORY;
-    }
-    MUGPU_TRY(class_of(r));
     hipStream_t sr = r->cls->sr;
     running.push_back(r);
     ++stats[1];
@@ -906,7 +925,7 @@ extern "C" int mugpu_server_progress(mugpu_server s, int wait, int* unfinished) 
 }
 
 extern "C" int mugpu_server_poll(mugpu_server s, int64_t* ids, int capacity, int* count) {
-  if (!s || !count) return MUGPU_EARG;
+  if (!s || !count || capacity < 0 || (capacity > 0 && !ids)) return MUGPU_EARG;
   const int n = std::min<int>(capacity, (int)s->done_ids.size());
   for (int i = 0; i < n; ++i) ids[i] = s->done_ids[i];
   s->done_ids.erase(s->done_ids.begin(), s->done_ids.begin() + n);

@@ -117,3 +117,68 @@ def test_cell_creation_and_initialise_fail_cleanly(capi):
         capi.check(lib.mugpu_cell_evaluate_stress_tangent(h))
     finally:
         lib.mugpu_cell_destroy(h)
+
+
+def _rve_request(res, seed):
+    rng = np.random.default_rng(seed)
+    N = int(np.prod(res))
+    E = np.where(rng.random(N) < 0.4, 100e9, 10e9)
+    nu = np.full(N, 0.3)
+    d = len(res)
+    a = rng.uniform(-0.01, 0.01, (d, d))
+    return res, E, nu, 0.5 * (a + a.T)
+
+
+def test_server_admission_fails_cleanly(capi):
+    """Admitting the first request of a resolution sets up its class (the
+    transform plan, the pixel list, the batch slots).  With each of those
+    allocations refused in turn, progress reports MUGPU_EMEMORY and the
+    request stays queued, admitted by nobody and holding nothing (its arena
+    block, pinned words and event go back); once the device is available
+    again the same server serves it, with the result of a server that never
+    failed, bit for bit."""
+    from muspectre_amd import serving
+    lib = capi.load()
+    res, E, nu, eps = _rve_request([32, 16, 48], 5)
+
+    ref_srv = serving.RVEServer(hbm_budget=1 << 30, keep_fields=True)
+    try:
+        rid = ref_srv.submit(res, E, nu, eps)
+        ref_srv.run()
+        ref = ref_srv.result(rid, fields=True)
+    finally:
+        ref_srv.close()
+    assert ref["ok"] and ref["converged"]
+
+    srv = serving.RVEServer(hbm_budget=1 << 30, keep_fields=True)
+    try:
+        rid = srv.submit(res, E, nu, eps)
+        refused = 0
+        for after in range(64):
+            capi.check(lib.mugpu_debug_fail_allocations(after))
+            try:
+                srv.progress()
+            except capi.MugpuError as e:
+                assert e.code == capi.EMEMORY and "injected" in str(e)
+                st = srv.stats()
+                assert st["admitted"] == 0 and st["peak_running"] == 0
+                refused += 1
+                continue
+            finally:
+                capi.check(lib.mugpu_debug_fail_allocations(-1))
+            break
+        else:
+            pytest.fail("the class set-up never completed")
+        assert refused >= 4  # plan tables and partials, pixel list, slots, control words
+        srv.run()
+        assert srv.poll() == [rid]
+        got = srv.result(rid, fields=True)
+        st = srv.stats()
+        assert st["admitted"] == 1 and st["finished"] == 1 and st["failed"] == 0
+    finally:
+        srv.close()
+    for k in ("status", "converged", "newton_iterations", "cg_iterations"):
+        assert got[k] == ref[k]
+    np.testing.assert_array_equal(got["strain"], ref["strain"])
+    np.testing.assert_array_equal(got["stress"], ref["stress"])
+    np.testing.assert_array_equal(got["mean_stress"], ref["mean_stress"])
