@@ -16,9 +16,12 @@
  *  src/fft/fftw_engine.cc:40-77).  Pointers named d_* are device pointers.
  *
  * precision: MUGPU_FP64 (plain path) or MUGPU_FP32 (operator stage in
- * single precision).  Threading: calls on one handle are not thread-safe;
- * all work runs on the library's stream, and the functions that return
- * host scalars synchronise it.
+ * single precision).  Threading: call the library from one thread at a
+ * time.  All handles share the library's stream, its pinned result words,
+ * its device scratch and its timing state, so calls that overlap from
+ * several threads race, whatever the handles; the functions that return
+ * host scalars synchronise the stream.  The text of mugpu_last_error() is
+ * that of the calling thread's last failed call.
  */
 #ifndef MUGPU_H
 #define MUGPU_H

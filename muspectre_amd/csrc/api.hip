@@ -18,7 +18,8 @@
 namespace mugpu {
 
 namespace {
-char g_err[2048] = "";
+// the message of the calling thread's last failed call
+thread_local char g_err[2048] = "";
 hipStream_t g_stream = nullptr;
 int g_device = -1;
 int g_maxlds = 65536;
@@ -51,10 +52,12 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
   return MUGPU_EHIP;
 }
 
-hipStream_t g_cur_stream = nullptr;
+// the stream the calling thread's launches go to (null: the library's
+// stream) and whether that thread is capturing a graph
+thread_local hipStream_t g_cur_stream = nullptr;
 hipStream_t stream() { return g_cur_stream ? g_cur_stream : g_stream; }
 void set_current_stream(hipStream_t s) { g_cur_stream = s; }
-bool g_capturing = false;
+thread_local bool g_capturing = false;
 void set_capturing(bool on) { g_capturing = on; }
 int max_lds_per_block() { return g_maxlds; }
 

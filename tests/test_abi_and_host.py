@@ -69,6 +69,31 @@ def test_no_gpu_fails_loudly():
         mu.fft.FFT([5, 5], 4).initialise()
 
 
+def test_error_text_is_per_thread():
+    """mugpu_last_error() gives the calling thread's last message: a failure
+    on another thread does not change it.  (Argument checks of
+    mugpu_server_footprint run before any device call.)"""
+    import threading
+    lib = ctypes.CDLL(os.path.join(ROOT, "muspectre_amd", "lib", "libmugpu.so"))
+    lib.mugpu_last_error.restype = ctypes.c_char_p
+    size = ctypes.c_size_t()
+
+    def fail(n):
+        res = (ctypes.c_int * 3)(n, 32, 32)
+        assert lib.mugpu_server_footprint(3, res, ctypes.byref(size)) != 0
+        return lib.mugpu_last_error().decode()
+
+    mine = fail(17)
+    assert "got 17" in mine
+    seen = []
+    t = threading.Thread(target=lambda: seen.append((lib.mugpu_last_error().decode(), fail(19))))
+    t.start()
+    t.join()
+    before, theirs = seen[0]
+    assert before == "" and "got 19" in theirs
+    assert lib.mugpu_last_error().decode() == mine
+
+
 def test_product_never_touches_the_oracle():
     """Only the build script names the oracle (it compiles the checker beside
     the product); no runtime file of the package imports or loads it."""
