@@ -25,11 +25,14 @@
  * tangent (mo_pk1_from_kirchhoff), for which no reference vector exists, is
  * fixed uniquely by two identities on random inputs: K(i,m,j,n) does not
  * depend on j, and F K over its first index equals the sums of C over its
- * last index.  Parity unpinned (nothing of that kind fixes them):
- * logm/expm at exactly repeated eigenvalues against Eigen's computeDirect
- * rounding (the mathematical function is matched to 1e-13); the reference's
- * own Newton/CG iteration counts at the benchmark configurations; HEP1
- * history over more steps than the two golden states.
+ * last index.  HEP1 over many steps is held, at every step of a
+ * loading/unloading path, to what its return mapping fixes for any state
+ * (deviatoric flow, Hencky stress of the new be, the yield surface, and
+ * be = F F^T on an elastic path).  Parity unpinned (nothing of that kind
+ * fixes them): logm/expm at exactly repeated eigenvalues against Eigen's
+ * computeDirect rounding (the mathematical function is matched to 1e-13);
+ * the reference's own Newton/CG iteration counts at the benchmark
+ * configurations.
  */
 #ifndef MUSPECTRE_ORACLE_H
 #define MUSPECTRE_ORACLE_H

@@ -1,6 +1,7 @@
 """CPU checks that pin the oracle's constitutive laws and tensor functions.
 
-Pins: the reference's golden values for HEP1 (two states) and expm;
+Pins: the reference's golden values for HEP1 (two states) and expm; what
+HEP1's return mapping fixes at every step of a many-step path;
 numpy.linalg.eigh for logm/expm on separated and repeated spectra; the
 explicit Curnier forms of the LE1 finite-strain P and K used by the
 reference (tests/test_goodies.hh:94-134); closed-form Hooke stresses
@@ -191,6 +192,90 @@ def test_hep1_consistent_tangent_is_derivative(oracle, seed):
     Kfd = _fd_tangent(lambda G: oracle.hep1_stress_and_tangent(3, E, nu, ty, H, G, Fp, be, 0.001,
                                                                False)[0], F)
     assert np.abs(K - Kfd).max() < 1e-7 * np.abs(Kfd).max()
+
+
+def _spd_logm(A):
+    w, V = np.linalg.eigh(0.5 * (A + A.T))
+    return (V * np.log(w)) @ V.T
+
+
+def _hep1_path(oracle, steps, amp, seed, E=70e9, nu=.33, ty=200e6, H=1e9):
+    """HEP1 over a loading path, its state cycled as the reference's state
+    fields cycle after each step (statefield.hh:318-326): F_prev and be take
+    the last step's values as old; the current eps_p buffer holds the value
+    of two steps back, which the step adds its increment to
+    (material_hyper_elasto_plastic1.hh:262).  Yields (F, F_prev_old, be_old,
+    eps_old, result of the step, eps_p increment) per step."""
+    rng = np.random.default_rng(seed)
+    Fprev, be = np.eye(3), np.eye(3)
+    eps = [0.0, 0.0]  # values of two steps back and of the last step
+    G = np.zeros((3, 3))
+    for _ in range(steps):
+        # a random walk of the distortion: loading and unloading, with and
+        # without rotation
+        G = G + amp * rng.standard_normal((3, 3))
+        F = np.eye(3) + G
+        out = oracle.hep1_eval(3, E, nu, ty, H, F, Fprev, be, eps[1], eps[0])
+        dgamma = out[4] - eps[0]
+        yield F, Fprev, be, eps[1], out, dgamma
+        Fprev, be = out[2], out[3]
+        eps = [eps[1], out[4]]
+
+
+def test_hep1_return_mapping_invariants_over_a_path(oracle):
+    """Sixteen steps of HEP1 with history, each checked against what the
+    return mapping (material_hyper_elasto_plastic1.hh:227-269) fixes exactly
+    for any state: F_prev becomes F; the flow direction is deviatoric, so
+    det(be_new) = det(f be_old f^T) with f = F F_prev_old^-1; the stress is
+    the Hencky stress of the new be, tau = lambda tr(e) I + 2 mu e with
+    e = log(be_new) / 2, and commutes with be_new; the increment is not
+    negative, and the equivalent stress sqrt(3/2 tau_d:tau_d) lies on the
+    updated yield surface tau_y0 + H (eps_old + dgamma) after a plastic step
+    and within the old one after an elastic step.  A wrong sign or factor in
+    the return, in the yield function or in the Hooke law fails one of
+    these; the path has both kinds of step."""
+    E, nu, ty, H = 70e9, .33, 200e6, 1e9
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    plastic = elastic = 0
+    for F, Fp, be, eps_old, out, dgamma in _hep1_path(oracle, 16, 0.002, 7, E, nu, ty, H):
+        tau, _, Fpc, bec, _ = out
+        assert np.array_equal(Fpc, F)
+        f = F @ np.linalg.inv(Fp)
+        bstar = f @ be @ f.T
+        assert abs(np.linalg.det(bec) / np.linalg.det(bstar) - 1) < 1e-12
+        e = 0.5 * _spd_logm(bec)
+        hencky = lam * np.trace(e) * np.eye(3) + 2 * mu * e
+        assert np.abs(tau - hencky).max() < 1e-9 * np.abs(tau).max()
+        assert np.abs(tau @ bec - bec @ tau).max() < 1e-9 * np.abs(tau).max()
+        td = tau - np.trace(tau) / 3 * np.eye(3)
+        teq = np.sqrt(1.5 * (td * td.T).sum())
+        assert dgamma >= 0.0
+        if dgamma > 0:
+            plastic += 1
+            assert abs(teq - (ty + H * (eps_old + dgamma))) < 1e-9 * ty
+        else:
+            elastic += 1
+            assert teq <= ty + H * eps_old
+    assert plastic >= 3 and elastic >= 3, (plastic, elastic)
+
+
+def test_hep1_elastic_path_be_is_left_cauchy_green(oracle):
+    """Without plastic flow be is the left Cauchy-Green tensor F F^T at
+    every step, whatever the path, and the stress is its Hencky stress; the
+    relative deformation f = F F_prev^-1 taken in the other order,
+    F_prev^-1 F, fails this from the second step on (the shears of the
+    golden states commute, so they cannot tell the orders apart)."""
+    E, nu, ty, H = 70e9, .33, 200e6, 1e9
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    for F, _, _, _, out, dgamma in _hep1_path(oracle, 8, 0.0002, 3, E, nu, ty, H):
+        tau, _, _, bec, epc = out
+        assert dgamma == 0.0 and epc == 0.0
+        b = F @ F.T
+        assert np.abs(bec - b).max() < 1e-13
+        e = 0.5 * _spd_logm(b)
+        assert np.abs(tau - (lam * np.trace(e) * np.eye(3) + 2 * mu * e)).max() < 1e-9 * np.abs(tau).max()
 
 
 def test_hep1_coded_tangent_is_j_independent(oracle):
