@@ -32,6 +32,32 @@ def test_capi_signatures_cover_header():
     assert declared == bound
 
 
+def test_capi_argument_lists_match_header():
+    """Every ctypes binding passes as many arguments as its declaration takes,
+    a pointer where the declaration takes one and a scalar of the same width
+    where it does not."""
+    import ctypes
+    from muspectre_amd import _capi
+    src = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "mugpu.h")).read(), flags=re.S)
+    decl = {}
+    for m in re.finditer(r"\b(mugpu_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", src, flags=re.S):
+        a = m.group(2).strip()
+        decl[m.group(1)] = [] if a in ("", "void") else [x.strip() for x in a.split(",")]
+    handles = {"mugpu_cell", "mugpu_fft", "mugpu_proj", "mugpu_server"}
+    widths = {"int": 4, "int64_t": 8, "size_t": 8, "double": 8, "float": 4}
+    for name, _, args in _capi.SIGNATURES:
+        params = decl[name]
+        assert len(params) == len(args), name
+        for c, t in zip(params, args):
+            words = c.replace("*", " * ").split()
+            base = [w for w in words if w not in ("const", "*")][0]
+            if "*" in words or base in handles:
+                assert t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer), (name, c)
+            else:
+                assert ctypes.sizeof(t) == widths[base] and t not in (ctypes.c_void_p,), (name, c)
+                assert (t in (ctypes.c_double, ctypes.c_float)) == (base in ("double", "float")), (name, c)
+
+
 def test_module_surface():
     import muspectre_amd as mu
     for name in ("Cell", "Formulation", "material", "solvers", "fft", "get_domain_ccoord",
