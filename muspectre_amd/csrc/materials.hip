@@ -656,7 +656,53 @@ int launch_le(const MaterialDev& m, MatArgs& a, dim3 grid) {
   return MUGPU_OK;
 }
 
+// the material stage of up to kServeMatSlots serving requests of one
+// resolution: per voxel the arithmetic of k_linear_elastic<D, LE4, small
+// strain> over the identity pixel list, P negated into the right-hand side
+// and the tangent rounded to FP32 as convert() does
+struct ServeMatArgs {
+  ServeMatSlot s[kServeMatSlots];
+};
+static_assert(sizeof(ServeMatArgs) + sizeof(int64_t) <= 4096, "kernel arguments of at most 4 KiB");
+template <int D>
+__global__ __launch_bounds__(256) void k_le4_serve_batch(ServeMatArgs a, int64_t n) {
+  constexpr int NC = D * D;
+  const ServeMatSlot& sl = a.s[blockIdx.y];
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  double E[NC], S[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) E[c] = sl.F[v * NC + c];
+  const double lam = sl.params[2 * v], mu = sl.params[2 * v + 1];
+  iso_contract<D>(lam, mu, E, S);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    sl.P[v * NC + c] = S[c];
+    sl.rhs[v * NC + c] = S[c] * -1.0;
+  }
+  sl.tan64[tan_off(v, 0, 2)] = lam;
+  sl.tan64[tan_off(v, 1, 2)] = mu;
+  sl.tan32[tan_off(v, 0, 2)] = (float)(lam * 1.0);
+  sl.tan32[tan_off(v, 1, 2)] = (float)(mu * 1.0);
+}
+
 }  // namespace
+
+int serve_material_batch(int dim, const ServeMatSlot* slots, int nslots, int64_t n) {
+  if (nslots < 1 || nslots > kServeMatSlots || n < 1 || (dim != 2 && dim != 3)) {
+    set_error("serve_material_batch: %d requests of %lld voxels in %dD", nslots, (long long)n, dim);
+    return MUGPU_EARG;
+  }
+  ServeMatArgs a{};
+  for (int i = 0; i < nslots; ++i) a.s[i] = slots[i];
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)nslots);
+  timing_begin(MUGPU_K_MATERIAL);
+  if (dim == 3) hipLaunchKernelGGL(k_le4_serve_batch<3>, grid, dim3(256), 0, stream(), a, n);
+  else hipLaunchKernelGGL(k_le4_serve_batch<2>, grid, dim3(256), 0, stream(), a, n);
+  timing_end(MUGPU_K_MATERIAL);
+  MUGPU_HIP(hipGetLastError());
+  return MUGPU_OK;
+}
 
 int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,
                   int tan_kind, double* tangent, int need_tangent) {

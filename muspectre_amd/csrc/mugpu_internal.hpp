@@ -283,4 +283,20 @@ struct MaterialDev {
 int eval_material(const MaterialDev& m, int dim, int form, const double* F, double* P,
                   int tan_kind, double* tangent, int need_tangent);
 
+// serving: the material stage of several requests of one resolution in one
+// launch (one request per blockIdx.y): small-strain per-voxel isotropic
+// elasticity (MaterialLinearElastic4 over all pixels), writing the stress P,
+// the right-hand side -P, the TAN_ISO tangent and its FP32 copy, bit for bit
+// what eval_material and convert write
+struct ServeMatSlot {
+  const double* F;
+  const double* params;  // (lambda, mu) per voxel
+  double* P;
+  double* rhs;
+  double* tan64;
+  float* tan32;
+};
+constexpr int kServeMatSlots = 64;  // requests per launch (passed by value)
+int serve_material_batch(int dim, const ServeMatSlot* slots, int nslots, int64_t n);
+
 }  // namespace mugpu

@@ -17,7 +17,9 @@
 // count).  Every slot carries its own scalars and stopping test, evaluated
 // on the device, so a slot that finishes is skipped by every later launch
 // and stops exactly where the same solve stops alone; new requests take
-// free slots between chunks (continuous batching).
+// free slots between chunks (continuous batching).  The material stage
+// (stress, tangent) of the requests of one resolution that reach it in the
+// same pass of the loop runs as one launch too (one request per blockIdx.y).
 //
 // Memory: one device arena of the HBM budget; a request is admitted (first
 // come, first served) when its whole footprint fits, and gives it back when
@@ -185,6 +187,7 @@ Layout layout_of(int dim, const int* res) {
 
 enum Stage {
   ST_QUEUED,
+  ST_EVAL_QUEUED,  // waiting for its class's batched material stage
   ST_WAIT_EVAL,    // stress, tangent, rhs = -G:P and |rhs|^2 queued
   ST_WAIT_BNORM,   // |rhs|^2 of the solve
   ST_WAIT_RESID,   // FP64 residual |b - A x|^2
@@ -259,7 +262,6 @@ struct ResClass {
     if (ev) (void)hipEventDestroy(ev);
     if (sb) (void)hipStreamDestroy(sb);
     if (sr) (void)hipStreamDestroy(sr);
-    if (d_iota) (void)hipFree(d_iota);
     if (d_slots) (void)hipFree(d_slots);
     if (d_ctl) (void)hipFree(d_ctl);
     if (h_slots) (void)hipHostFree(h_slots);
@@ -271,7 +273,6 @@ struct ResClass {
   hipStream_t sb = nullptr, sr = nullptr;
   FftPlan plan;
   int64_t N = 0;
-  int* d_iota = nullptr;  // pixel list 0 .. N-1 of the material kernel
   BatchSlot* d_slots = nullptr;
   BatchSlot* h_slots = nullptr;  // pinned staging
   int* d_ctl = nullptr;
@@ -279,6 +280,7 @@ struct ResClass {
   int* h_ctl_down = nullptr;  // pinned
   std::vector<Request*> slot_req;
   std::deque<Request*> waiting;
+  std::vector<Request*> eval_ready;  // at the material stage, not yet launched
   int nlaunch = 0;  // slots in the chunk in flight
   bool in_flight = false;
   hipEvent_t ev = nullptr;
@@ -353,6 +355,8 @@ struct mugpu_server_s {
   int issue(Request* r);  // queue the next device stage of r (r->st tells which)
   int advance(Request* r);
   int start_eval(Request* r);
+  int flush_evals(ResClass* c);
+  int project_rhs(Request* r);
   int start_resid(Request* r);
   int finish(Request* r);
   int after_inner(Request* r, bool ran);
@@ -386,10 +390,6 @@ int mugpu_server_s::class_of(Request* r) {
   c->N = r->N;
   MUGPU_TRY(plan_init(c->plan, r->dim, r->res, r->nc, MUGPU_FP32));
   const int cap = cfg.max_batch;
-  std::vector<int> iota((size_t)c->N);
-  for (int64_t i = 0; i < c->N; ++i) iota[(size_t)i] = (int)i;
-  MUGPU_TRY(dev_alloc(&c->d_iota, sizeof(int) * (size_t)c->N, "a resolution class's pixel list"));
-  MUGPU_HIP(hipMemcpy(c->d_iota, iota.data(), sizeof(int) * (size_t)c->N, hipMemcpyHostToDevice));
   MUGPU_TRY(dev_alloc(&c->d_slots, sizeof(BatchSlot) * cap, "the batch slots"));
   MUGPU_TRY(dev_alloc(&c->d_ctl, sizeof(int) * CTL_COUNT * cap, "the batch control words"));
   MUGPU_HIP(hipHostMalloc((void**)&c->h_slots, sizeof(BatchSlot) * cap, hipHostMallocDefault));
@@ -480,23 +480,46 @@ int mugpu_server_s::admit() {
   return MUGPU_OK;
 }
 
-// one Newton iteration begins: stress and tangent, rhs = -G:P, |rhs|
-// (solvers.cc:165-176)
+// one Newton iteration begins (solvers.cc:165-176): the request waits for
+// its class's batched material stage, which flush_evals launches once per
+// progress pass for every request that reached it
 int mugpu_server_s::start_eval(Request* r) {
-  hipStream_t sr = r->cls->sr;
   const int newton_bound = r->maxiter;
   if (r->newt_iter >= newton_bound || r->has_converged) return finish(r);
-  set_current_stream(sr);
-  MaterialDev m;
-  m.family = MUGPU_LE4;
-  m.d_pix = r->cls->d_iota;
-  m.d_params = r->params;
-  m.ppn = 2;
-  m.n_pixels = r->N;
-  MUGPU_TRY(eval_material(m, r->dim, MUGPU_SMALL_STRAIN, r->F, r->P, TAN_ISO, r->tan64, 1));
-  MUGPU_TRY(convert(MUGPU_FP64, r->tan64, MUGPU_FP32, r->tan32,
-                    tangent_values(TAN_ISO, r->dim, r->N), 1.0));
-  MUGPU_TRY(convert(MUGPU_FP64, r->P, MUGPU_FP64, r->rhs, r->nd, -1.0));
+  r->st = ST_EVAL_QUEUED;
+  r->cls->eval_ready.push_back(r);
+  return MUGPU_OK;
+}
+
+// stress and tangent of every request of c at the material stage, in
+// launches of up to kServeMatSlots requests (the same per-voxel arithmetic
+// as eval_material followed by the FP32 tangent copy and rhs = -P), then each
+// request's rhs = -G:P and |rhs|^2.  A request leaves eval_ready once its
+// stage is queued; after an error the rest stay there.
+int mugpu_server_s::flush_evals(ResClass* c) {
+  if (c->eval_ready.empty()) return MUGPU_OK;
+  set_current_stream(c->sr);
+  const size_t total = c->eval_ready.size();
+  for (size_t i0 = 0; i0 < total; i0 += kServeMatSlots) {
+    const int n = (int)std::min<size_t>(kServeMatSlots, total - i0);
+    ServeMatSlot sl[kServeMatSlots];
+    for (int i = 0; i < n; ++i) {
+      const Request* r = c->eval_ready[i0 + i];
+      sl[i] = ServeMatSlot{r->F, r->params, r->P, r->rhs, r->tan64, r->tan32};
+    }
+    MUGPU_TRY(serve_material_batch(c->dim, sl, n, c->N));
+  }
+  size_t done = 0;
+  int rc = MUGPU_OK;
+  for (; done < total && rc == MUGPU_OK; ++done) rc = project_rhs(c->eval_ready[done]);
+  if (rc != MUGPU_OK) --done;
+  c->eval_ready.erase(c->eval_ready.begin(), c->eval_ready.begin() + (std::ptrdiff_t)done);
+  return rc;
+}
+
+// rhs = -G:P in FP64 (P negated by the material stage) and |rhs|^2
+int mugpu_server_s::project_rhs(Request* r) {
+  set_current_stream(r->cls->sr);
   OpSpec op;
   op.in_mode = IN_PLAIN;
   op.in = r->rhs;
@@ -886,7 +909,9 @@ extern "C" int mugpu_server_progress(mugpu_server s, int wait, int* unfinished) 
   // requests whose stage completed
   for (size_t i = 0; i < s->running.size(); ++i) {
     Request* r = s->running[i];
-    if (r->st == ST_BATCH_QUEUE || r->st == ST_BATCH || r->st == ST_DONE) continue;
+    if (r->st == ST_BATCH_QUEUE || r->st == ST_BATCH || r->st == ST_DONE ||
+        r->st == ST_EVAL_QUEUED)
+      continue;
     const hipError_t q = hipEventQuery(r->ev);
     if (q == hipErrorNotReady) continue;
     MUGPU_HIP(q);
@@ -912,6 +937,12 @@ extern "C" int mugpu_server_progress(mugpu_server s, int wait, int* unfinished) 
                                   [](Request* r) { return r->st == ST_DONE; }),
                    s->running.end());
   MUGPU_TRY(s->admit());
+  // the material stages reached in this pass, batched per resolution
+  for (auto& kv : s->classes) {
+    if (kv.second->eval_ready.empty()) continue;
+    MUGPU_TRY(s->flush_evals(kv.second.get()));
+    moved = true;
+  }
   if (wait && !moved) {
     // block on the oldest outstanding device work
     for (Request* r : s->running)
