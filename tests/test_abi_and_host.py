@@ -120,6 +120,17 @@ def test_error_text_is_per_thread():
     assert lib.mugpu_last_error().decode() == mine
 
 
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N != 1 exits with status 2 and says why, before
+    it touches a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
+
+
 def test_product_never_touches_the_oracle():
     """Only the build script names the oracle (it compiles the checker beside
     the product); no runtime file of the package imports or loads it."""
