@@ -151,10 +151,15 @@ __device__ __forceinline__ void project_point(c_t<T>* f, int stride, const T* xi
 
 // unit wave vector of Fourier point c (first DIM coordinates) of the grid
 // res with lengths L; false for k = 0 and, under the even-grid policy, for a
-// Nyquist index on an even axis (the operator is zero there)
+// Nyquist index on an even axis (the operator is zero there).  |xi|^2 is
+// summed without contraction into fused multiply-adds: left to the compiler,
+// whether it contracts depends on the kernel the function is inlined into,
+// and the serving and single-cell first-axis kernels then differ in the last
+// bit of xi wherever 1/L is inexact (lengths like 7 or 5)
 template <typename T, int DIM>
 __device__ __forceinline__ bool unit_wave_vector(const int* c, const int* res, const double* L,
                                                  int allow_even, T* xi) {
+#pragma clang fp contract(off)
   T n2 = 0;
   bool zero = true, nyq = false;
 #pragma unroll

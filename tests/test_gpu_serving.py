@@ -100,6 +100,35 @@ def test_request_equals_cell_newton_cg(µ, serving, res):
     assert np.abs(r["mean_stress"] - mean).max() <= 1e-13 * np.abs(mean).max()
 
 
+@pytest.mark.parametrize("res,lengths", [([32, 16, 48], [2.0, 1.0, 0.5]), ([64, 32], [7.0, 5.0])],
+                         ids=["32x16x48", "64x32"])
+def test_request_lengths_equal_cell(µ, serving, res, lengths):
+    """A request with unequal cell lengths (the projection's wave vectors
+    scale with 1/L) computes exactly what newton_cg computes on a Cell of the
+    same lengths, and not what it computes on the unit cell."""
+    q = _request(res, 11)
+    d = len(res)
+    srv = serving.RVEServer(hbm_budget=2 << 30, keep_fields=True)
+    try:
+        rid = srv.submit(q["res"], q["E"], q["nu"], q["strain"], lengths=lengths)
+        unit = srv.submit(q["res"], q["E"], q["nu"], q["strain"])
+        srv.run()
+        r, u = srv.result(rid, fields=True), srv.result(unit, fields=True)
+    finally:
+        srv.close()
+    cell = µ.Cell(res, lengths, µ.Formulation.small_strain, precision="fp32", allow_even=True)
+    mat = getattr(µ.material, "MaterialLinearElastic4_{}d".format(d)).make(cell, "rve")
+    mat.add_pixels_by_index(np.arange(q["E"].size), np.stack([q["E"], q["nu"]], axis=1))
+    solver = µ.solvers.SolverCG(cell, 1e-6, 2000, False)
+    ref = µ.solvers.newton_cg(cell, [q["strain"]], solver, 1e-5, 1e-10, 0)[0]
+    nc = d * d
+    assert r["ok"] and r["converged"] == ref.success
+    assert r["newton_iterations"] == ref.nb_it and r["cg_iterations"] == ref.nb_fev
+    np.testing.assert_array_equal(r["strain"], np.asarray(ref.grad).reshape(-1, nc))
+    np.testing.assert_array_equal(r["stress"], np.asarray(ref.stress).reshape(-1, nc))
+    assert np.abs(r["strain"] - u["strain"]).max() > 1e-3 * np.abs(u["strain"]).max()
+
+
 MIX = [[64, 64], [32, 32, 32], [128, 128], [48, 48, 48], [32, 32, 32], [64, 64],
        [48, 48, 48], [32, 32, 32], [16, 16, 16], [32, 32, 32], [64, 64], [96, 32, 16]]
 
