@@ -143,6 +143,21 @@ def test_alone_equals_mix(serving):
     assert st["peak_running"] > 1 and st["batched_slots"] > st["batched_launches"]
 
 
+def test_many_requests_of_one_resolution(serving):
+    """70 requests of one small resolution admitted together: the material
+    stage of the first pass runs as two launches (64 requests and 6), the
+    inner iterations in batches of up to 64 slots, and every request still
+    computes exactly its alone result."""
+    reqs = [_request([16, 16, 16], 200 + i) for i in range(70)]
+    alone = _alone(serving, reqs[:6]) + _alone(serving, reqs[64:])
+    mixed, st = _serve(serving, reqs, max_batch=64, iters_per_launch=4)
+    for a, b in zip(alone, mixed[:6] + mixed[64:]):
+        assert a["ok"]
+        _same(a, b)
+    assert all(m["ok"] for m in mixed)
+    assert st["peak_running"] == 70 and st["batched_slots"] > st["batched_launches"]
+
+
 def test_failure_is_isolated(serving):
     reqs = [_request(r, 40 + i) for i, r in enumerate(MIX[:6])]
     reqs[2] = dict(reqs[2], maxiter=3)
