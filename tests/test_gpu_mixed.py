@@ -68,11 +68,13 @@ def test_c1_disc_fp32_refined(µ, oracle):
     assert solver.get_refinements() >= 2
 
 
+@pytest.mark.parametrize("driver", ["newton_cg", "de_geus"])
 @pytest.mark.parametrize("res", [[9, 11, 7], [16, 32, 16]], ids=lambda r: "x".join(map(str, r)))
 @pytest.mark.parametrize("form", ["small", "finite"])
-def test_3d_two_phase_fp32_refined(µ, oracle, form, res):
+def test_3d_two_phase_fp32_refined(µ, oracle, form, res, driver):
     """Runtime-length and compiled-length axes (the latter with the x update
-    on the next first pass and the even policy)."""
+    on the next first pass and the even policy), under both drivers (de_geus
+    takes its first right-hand side from -G:K:dF, solvers.cc:353-363)."""
     even = any(n % 2 == 0 for n in res)
     g = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
     mask = ((g[0] - 4) ** 2 + (g[1] - 5) ** 2 + (g[2] - 3) ** 2) < 9
@@ -85,14 +87,15 @@ def test_3d_two_phase_fp32_refined(µ, oracle, form, res):
     s.add_pixels_by_index(idx[~mask.ravel()])
     steps = [np.array([[.01, .002, 0], [.002, 0, 0], [0, 0, -.005]]) * k for k in (1, 2)]
     solver = µ.solvers.SolverCG(cell, 1e-8, 400, 0)
-    rs = µ.solvers.newton_cg(cell, steps, solver, 1e-5, 1e-10, 0)
+    rs = getattr(µ.solvers, driver)(cell, steps, solver, 1e-5, 1e-10, 0)
     oc = oracle.Cell(res, [1., 1.2, .8], oracle.SMALL_STRAIN if form == "small" else oracle.FINITE_STRAIN,
                      allow_even=even)
     a = oc.add_material(oracle.LE1, 100e9, .3)
     b = oc.add_material(oracle.LE1, 10e9, .3)
     for i in range(mask.size):
         oc.add_pixel(a if mask.flat[i] else b, i)
-    ro = oc.newton_cg(steps, 1e-8, 400, 1e-5, 1e-10)
+    ro = getattr(oc, driver)(steps, 1e-8, 400, 1e-5, 1e-10)
+    assert len(rs) == len(ro) == 2
     for r, o in zip(rs, ro):
         assert r.success and r.nb_it == o["nb_it"]
         assert _rel(_homog(r.stress, 9), _homog(o["stress"], 9)) < TOL
