@@ -96,8 +96,9 @@ int mugpu_device_info(char *name, int name_len, int *cu_count,
 int mugpu_alloc(size_t bytes, void **d_ptr);
 /* Test hook for the failure paths of the library's own device allocations
  * (no reference counterpart; for tests of out-of-memory handling): from now
- * on the allocations of cells, projections, the mixed-precision work vectors
- * and the bandwidth probes (not mugpu_alloc) succeed `after` more times, and
+ * on the allocations of cells, transform plans, projections, the
+ * mixed-precision work vectors, serving resolution classes and the
+ * bandwidth probes (not mugpu_alloc) succeed `after` more times, and
  * every later one fails with MUGPU_EMEMORY as if the device were full, until
  * the hook is called with after < 0 (the default: no injected failure). */
 int mugpu_debug_fail_allocations(int after);
@@ -436,6 +437,11 @@ int mugpu_timing_class(int kernel_class, int64_t *launches, double *total_ms);
 #define MUGPU_PASS_AXIS0_PLAIN 4  /* first-axis c2c of the engine transforms  */
 int mugpu_fft_tune_pass(int dim, const int *resolutions, int precision, int pass,
                         int variant, int iso, int reps, double *avg_ms);
+/* Test hook (no reference counterpart; for tests that run every variant a
+ * tuned table could choose): plans created from now on use `variant` of
+ * `pass` (MUGPU_PASS_*) in `precision` on every axis length where that
+ * variant exists, in place of the tuned one; variant < 0 restores the table. */
+int mugpu_debug_force_variant(int pass, int precision, int variant);
 /* streaming device copy of `bytes` (multiple of 16), the bandwidth
  * reference for the roofline and for the HBM counter calibration */
 int mugpu_copy(void *d_dst, const void *d_src, size_t bytes);

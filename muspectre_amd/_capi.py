@@ -107,6 +107,7 @@ SIGNATURES = [
     ("mugpu_copy_bandwidth", _int, [ctypes.c_size_t, _int, _dp]),
     ("mugpu_line_bandwidth", _int, [_int, _int, ctypes.c_int64, _int, _int, _int, _dp]),
     ("mugpu_debug_fail_allocations", _int, [_int]),
+    ("mugpu_debug_force_variant", _int, [_int, _int, _int]),
 ]
 NOALLOC_MAX_VALUES = 2796000  # MUGPU_NOALLOC_MAX_VALUES
 K_LAST_R2C, K_MIDDLE, K_AXIS0, K_LAST_C2R, K_MATERIAL, K_CG_UPDATE = range(6)

@@ -461,6 +461,17 @@ int mugpu::dev_alloc(void** p, size_t bytes, const char* what) {
   return MUGPU_OK;
 }
 
+extern "C" int mugpu_debug_force_variant(int pass, int precision, int variant) {
+  if (pass < 0 || pass >= FV_COUNT || (precision != MUGPU_FP64 && precision != MUGPU_FP32) ||
+      variant > 7) {
+    set_error("mugpu_debug_force_variant: pass %d, precision %d, variant %d out of range", pass,
+              precision, variant);
+    return MUGPU_EARG;
+  }
+  force_variant(pass, precision, variant < 0 ? -1 : variant);
+  return MUGPU_OK;
+}
+
 extern "C" int mugpu_debug_fail_allocations(int after) {
   g_fail_alloc_after.store(after < 0 ? -1 : after);
   return MUGPU_OK;

@@ -64,7 +64,19 @@ struct FastTuned {
 #include "fft_tuned.inc"
 }  // namespace
 
+// variants forced by mugpu_debug_force_variant, per precision (0 FP64,
+// 1 FP32) and pass; -1: the tuned table
+namespace {
+int g_forced[2][FV_COUNT] = {{-1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1}};
+}  // namespace
+
+void force_variant(int pass, int prec, int variant) {
+  g_forced[prec == MUGPU_FP64 ? 0 : 1][pass] = variant;
+}
+
 int fast_variant(int pass, int n, int dim, int prec, int iso) {
+  const int forced = g_forced[prec == MUGPU_FP64 ? 0 : 1][pass];
+  if (forced >= 0 && fast_variant_valid(pass, n, dim, prec, forced)) return forced;
   for (const FastTuned& t : kFastTuned)
     if (t.pass == pass && t.n == n && t.dim == dim && t.prec == prec && t.iso == iso &&
         fast_variant_valid(pass, n, dim, prec, t.variant))

@@ -129,6 +129,9 @@ enum FastVariantPass { FV_R2C = 0, FV_STRIDED1 = 1, FV_AXIS0 = 2, FV_C2R = 3, FV
 // variant of a pass for an axis length, dimension, precision and tangent
 // class, from the tuned table (fft_tuned.inc)
 int fast_variant(int pass, int n, int dim, int prec, int iso);
+// make fast_variant return `variant` for a pass and precision wherever that
+// variant exists (-1: the tuned table again); plans read it when created
+void force_variant(int pass, int prec, int variant);
 // whether a pass variant exists (and differs from variant 0's tile) for
 // this length, dimension and precision
 bool fast_variant_valid(int pass, int n, int dim, int prec, int variant);
