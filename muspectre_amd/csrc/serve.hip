@@ -153,9 +153,10 @@ Layout layout_of(int dim, const int* res) {
   const int nc = dim * dim;
   const int64_t nd = N * nc;
   const int64_t rows = N / res[dim - 1];
-  // c2r workgroups of the FP32 operator (one p.Ap partial each)
-  const int rb = fast_rows_per_block(
-      res[dim - 1], dim, fast_variant(FV_C2R, res[dim - 1], dim, MUGPU_FP32, 0));
+  // c2r workgroups of the FP32 operator (one p.Ap partial each), counted for
+  // the variant with the most (half the rows), so that the buffer holds the
+  // partials whichever variant the class's plan runs
+  const int rb = fast_rows_per_block(res[dim - 1], dim, 2);
   const int64_t nz = (rows + rb - 1) / rb;
   Layout l{};
   size_t o = 0;
