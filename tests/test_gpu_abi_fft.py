@@ -97,19 +97,23 @@ PROJ_CASES = [([5, 5], [5., 5.]), ([3, 5], [1.3, 2.1]), ([7, 7, 7], [7., 7., 7.]
 
 @pytest.mark.parametrize("res,lengths", PROJ_CASES, ids=lambda v: "x".join(map(str, v)))
 @pytest.mark.parametrize("kind", [0, 1, 2], ids=["small", "finite_fast", "finite"])
-def test_projection_vs_oracle(capi, oracle, res, lengths, kind):
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+def test_projection_vs_oracle(capi, oracle, res, lengths, kind, prec):
+    """In the engine's precision: FP64 to 1e-13, FP32 (field, transforms and
+    operator in single precision) to 2e-6 of the largest value."""
     lib = capi.load()
     d = len(res)
     nc = d * d
     rng = np.random.default_rng(7)
     f = rng.standard_normal((int(np.prod(res)), nc))
     ref = oracle.apply_projection(kind, res, lengths, f)
-    p = _projection(capi, res, lengths, kind)
+    p = _projection(capi, res, lengths, kind, prec=prec)
+    dt = np.float64 if prec == 0 else np.float32
     try:
-        buf = DeviceBuffer.from_array(f)
+        buf = DeviceBuffer.from_array(f.astype(dt))
         capi.check(lib.mugpu_projection_apply(p, buf.ptr))
-        out = buf.download(f.shape)
-        assert np.abs(out - ref).max() <= 1e-13 * np.abs(ref).max()
+        out = buf.download(f.shape, dt)
+        assert np.abs(out - ref).max() <= (1e-13 if prec == 0 else 2e-6) * np.abs(ref).max()
         # get_operator equals the oracle's stored operator
         kp = int(np.prod(list(res[:-1]) + [res[-1] // 2 + 1]))
         nop = d if kind == 1 else nc * nc
