@@ -69,12 +69,14 @@ def test_c1_disc_fp32_refined(µ, oracle):
 
 
 @pytest.mark.parametrize("driver", ["newton_cg", "de_geus"])
-@pytest.mark.parametrize("res", [[9, 11, 7], [16, 32, 16]], ids=lambda r: "x".join(map(str, r)))
+@pytest.mark.parametrize("res", [[9, 11, 7], [16, 32, 16], [16, 9, 32], [32, 24, 15]],
+                         ids=lambda r: "x".join(map(str, r)))
 @pytest.mark.parametrize("form", ["small", "finite"])
 def test_3d_two_phase_fp32_refined(µ, oracle, form, res, driver):
-    """Runtime-length and compiled-length axes (the latter with the x update
-    on the next first pass and the even policy), under both drivers (de_geus
-    takes its first right-hand side from -G:K:dF, solvers.cc:353-363)."""
+    """Runtime-length axes, compiled-length axes (the x update on the next
+    first pass, the even policy) and grids that mix the two, under both
+    drivers (de_geus takes its first right-hand side from -G:K:dF,
+    solvers.cc:353-363)."""
     even = any(n % 2 == 0 for n in res)
     g = np.meshgrid(*[np.arange(n) for n in res], indexing="ij")
     mask = ((g[0] - 4) ** 2 + (g[1] - 5) ** 2 + (g[2] - 3) ** 2) < 9
