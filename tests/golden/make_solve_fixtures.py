@@ -12,6 +12,11 @@ oracle (the CPU restatement of the reference's FFTW-double Newton-CG path).
   c5_128_newton1.npz  C5's material at 128^3: LE4 with lognormal E and
                       uniform nu per voxel, shear eps_xy = .01, the first
                       Newton iteration as for C3
+  c3_243_newton1.npz  C3 at 243^3 = 3^5, the odd grid next to the benchmark's
+                      (the reference's projection, no even-grid policy; the
+                      runtime-length FFT passes on the GPU), mask of the
+                      benchmark's generator at this size, the first Newton
+                      iteration as for C3
 
 Each file holds the inputs that are not regenerated bit for bit elsewhere
 (the C3 phase mask), the iteration counts, the stopping-test records of
@@ -20,8 +25,9 @@ stress of 1024 sampled voxels; the first-Newton-iteration fixtures also
 hold the oracle's own rounding spread (the same solve with 3 and 5 threads,
 i.e. another summation order of every dot product: the iteration counts
 and the largest |dx|/|x|).  Run from the repository root:
-    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5]
-(needs ~32 GB of memory for c3; OMP threads from MO_THREADS, default all).
+    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5] [c3_243]
+(needs ~32 GB of memory for c3 and c3_243; OMP threads from MO_THREADS,
+default all).
 """
 import os
 import sys
@@ -92,12 +98,12 @@ def first_newton(name, oc, F0, N, seed, extra):
         P0_sample=P0[idx], P1_sample=P1[idx], **extra)
 
 
-def c3():
-    n = 256
+def c3(n=256):
     res = [n] * 3
     N = n ** 3
-    mask = grf_mask(n, 16.0, 2026)
-    oc = O.Cell(res, [1.0] * 3, O.FINITE_STRAIN, allow_even=True)
+    # the benchmark's mask (correlation length 16 voxels at 256^3)
+    mask = grf_mask(n, 16.0 * n / 256.0, 2026)
+    oc = O.Cell(res, [1.0] * 3, O.FINITE_STRAIN, allow_even=n % 2 == 0)
     soft = oc.add_material(O.HEP1_CT, 70e9, .33, 200e6, 1e9)
     hard = oc.add_material(O.HEP1_CT, 210e9, .33, 800e6, 1e9)
     flat = mask.ravel()
@@ -106,7 +112,7 @@ def c3():
     oc.initialise()
     H1 = np.diag([0.0, 0.0, 0.005])
     F0 = (np.eye(3) + H1).ravel(order="F")
-    first_newton("c3_256_newton1", oc, F0, N, 1,
+    first_newton("c3_{}_newton1".format(n), oc, F0, N, 1,
                  {"mask_bits": np.packbits(flat), "res": np.array(res), "load": H1})
 
 
@@ -163,4 +169,4 @@ if __name__ == "__main__":
     O.set_threads(int(os.environ.get("MO_THREADS", os.cpu_count() or 1)))
     which = sys.argv[1:] or ["c2", "c5", "c3"]
     for w in which:
-        {"c3": c3, "c2": c2, "c5": c5}[w]()
+        {"c3": c3, "c2": c2, "c5": c5, "c3_243": lambda: c3(243)}[w]()

@@ -4,6 +4,8 @@ the oracle's solve (tests/golden/make_solve_fixtures.py writes them):
   C3 256^3  finite strain, two HyperElastoPlastic1 phases (consistent
             tangent), Gaussian-random-field mask, first load step: the first
             Newton iteration (newton_maxiter = 1, as the benchmark times it)
+  C3 243^3  the same on the odd grid next to it (3^5: the reference's
+            projection without the even-grid policy, runtime-length passes)
   C2 128^3  small strain LE1, 40 periodic spheres, the shear load case
             (newton_cg to convergence)
   C5 128^3  C5's material (LE4, lognormal E, uniform nu per voxel) and load,
@@ -59,9 +61,9 @@ def _rel(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
 
 
-def _c3_cell(µ, prec, hard):
-    cell = µ.Cell([256] * 3, [1.0] * 3, µ.Formulation.finite_strain, precision=prec,
-                  allow_even=True)
+def _c3_cell(µ, prec, hard, n=256):
+    cell = µ.Cell([n] * 3, [1.0] * 3, µ.Formulation.finite_strain, precision=prec,
+                  allow_even=n % 2 == 0)
     mk = µ.material.MaterialHyperElastoPlastic1_3d.make
     soft = mk(cell, "soft", 70e9, .33, 200e6, 1e9, consistent_tangent=True)
     hm = mk(cell, "hard", 210e9, .33, 800e6, 1e9, consistent_tangent=True)
@@ -117,6 +119,19 @@ def test_c3_256_first_newton_iteration(µ, prec):
     cell = _c3_cell(µ, prec, hard)
     F0 = (np.eye(3) + fx["load"]).ravel(order="F")
     _first_newton(µ, fx, cell, F0, prec, "c3 256^3")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_c3_243_first_newton_iteration(µ, prec):
+    """C3 on the odd grid next to the benchmark's, 243^3 = 3^5: the
+    reference's own projection (no even-grid policy) and the runtime-length
+    FFT passes at full scale."""
+    fx = _load("c3_243_newton1.npz")
+    n = 243
+    hard = np.unpackbits(fx["mask_bits"], count=n ** 3).astype(bool)
+    cell = _c3_cell(µ, prec, hard, n)
+    F0 = (np.eye(3) + fx["load"]).ravel(order="F")
+    _first_newton(µ, fx, cell, F0, prec, "c3 243^3")
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
