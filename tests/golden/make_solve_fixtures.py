@@ -12,6 +12,8 @@ oracle (the CPU restatement of the reference's FFTW-double Newton-CG path).
   c5_128_newton1.npz  C5's material at 128^3: LE4 with lognormal E and
                       uniform nu per voxel, shear eps_xy = .01, the first
                       Newton iteration as for C3
+  c2_125_shear.npz    C2 at 125^3 = 5^3, the odd grid next to the benchmark's
+                      (spheres of the benchmark's generator at this size)
   c3_243_newton1.npz  C3 at 243^3 = 3^5, the odd grid next to the benchmark's
                       (the reference's projection, no even-grid policy; the
                       runtime-length FFT passes on the GPU), mask of the
@@ -25,7 +27,7 @@ stress of 1024 sampled voxels; the first-Newton-iteration fixtures also
 hold the oracle's own rounding spread (the same solve with 3 and 5 threads,
 i.e. another summation order of every dot product: the iteration counts
 and the largest |dx|/|x|).  Run from the repository root:
-    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5] [c3_243]
+    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5] [c3_243] [c2_125]
 (needs ~32 GB of memory for c3 and c3_243; OMP threads from MO_THREADS,
 default all).
 """
@@ -116,12 +118,11 @@ def c3(n=256):
                  {"mask_bits": np.packbits(flat), "res": np.array(res), "load": H1})
 
 
-def c2():
-    n = 128
+def c2(n=128):
     mask = spheres_mask(n, 40, 2026).ravel()
     res = [n] * 3
     N = n ** 3
-    oc = O.Cell(res, [1.0] * 3, O.SMALL_STRAIN, allow_even=True)
+    oc = O.Cell(res, [1.0] * 3, O.SMALL_STRAIN, allow_even=n % 2 == 0)
     inc = oc.add_material(O.LE1, 100e9, .3)
     mat = oc.add_material(O.LE1, 10e9, .3)
     for i in range(N):
@@ -132,12 +133,13 @@ def c2():
     t = time.time()
     r = oc.newton_cg([e], 1e-8, 4000, 1e-5, 1e-10)[0]
     rec = O.cg_records()
-    print("c2_128_shear: {} Newton, {} CG iterations, {:.0f} s".format(r["nb_it"], r["nb_fev"],
-                                                                    time.time() - t), flush=True)
+    name = "c2_{}_shear".format(n)
+    print("{}: {} Newton, {} CG iterations, {:.0f} s".format(name, r["nb_it"], r["nb_fev"],
+                                                             time.time() - t), flush=True)
     idx = samples(N, 2)
     g, s = r["grad"], r["stress"]
     np.savez_compressed(
-        os.path.join(HERE, "c2_128_shear.npz"), res=np.array(res), load=e, nb_it=r["nb_it"],
+        os.path.join(HERE, name + ".npz"), res=np.array(res), load=e, nb_it=r["nb_it"],
         nb_fev=r["nb_fev"], success=r["success"], cg_records=rec, grad_norm=np.linalg.norm(g),
         stress_norm=np.linalg.norm(s), grad_mean=g.mean(0), stress_mean=s.mean(0),
         sample_index=idx, grad_sample=g[idx], stress_sample=s[idx],
@@ -169,4 +171,4 @@ if __name__ == "__main__":
     O.set_threads(int(os.environ.get("MO_THREADS", os.cpu_count() or 1)))
     which = sys.argv[1:] or ["c2", "c5", "c3"]
     for w in which:
-        {"c3": c3, "c2": c2, "c5": c5, "c3_243": lambda: c3(243)}[w]()
+        {"c3": c3, "c2": c2, "c5": c5, "c3_243": lambda: c3(243), "c2_125": lambda: c2(125)}[w]()
