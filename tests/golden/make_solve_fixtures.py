@@ -14,6 +14,7 @@ oracle (the CPU restatement of the reference's FFTW-double Newton-CG path).
                       Newton iteration as for C3
   c2_125_shear.npz    C2 at 125^3 = 5^3, the odd grid next to the benchmark's
                       (spheres of the benchmark's generator at this size)
+  c5_125_newton1.npz  C5's material and load at 125^3 (odd grid)
   c3_243_newton1.npz  C3 at 243^3 = 3^5, the odd grid next to the benchmark's
                       (the reference's projection, no even-grid policy; the
                       runtime-length FFT passes on the GPU), mask of the
@@ -27,7 +28,7 @@ stress of 1024 sampled voxels; the first-Newton-iteration fixtures also
 hold the oracle's own rounding spread (the same solve with 3 and 5 threads,
 i.e. another summation order of every dot product: the iteration counts
 and the largest |dx|/|x|).  Run from the repository root:
-    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5] [c3_243] [c2_125]
+    python tests/golden/make_solve_fixtures.py [c3] [c2] [c5] [c3_243] [c2_125] [c5_125]
 (needs ~32 GB of memory for c3 and c3_243; OMP threads from MO_THREADS,
 default all).
 """
@@ -146,24 +147,23 @@ def c2(n=128):
         inclusion_fraction=mask.mean())
 
 
-def c5():
-    """C5's material and load at 128^3, first Newton iteration (on a linear
+def c5(n=128):
+    """C5's material and load at n^3, first Newton iteration (on a linear
     problem the second Newton iteration asks CG to reduce a residual that
     is already at the operator's rounding level by a further 1e-8)."""
-    n = 128
     N = n ** 3
     # the benchmark's generator at 128^3 (bench.py Workload c5)
     rng = np.random.default_rng(2026)
     E = np.exp(np.log(50e9) + 0.8 * rng.standard_normal(N))
     nu = rng.uniform(0.15, 0.45, N)
-    oc = O.Cell([n] * 3, [1.0] * 3, O.SMALL_STRAIN, allow_even=True)
+    oc = O.Cell([n] * 3, [1.0] * 3, O.SMALL_STRAIN, allow_even=n % 2 == 0)
     m = oc.add_material(O.LE4)
     for i in range(N):
         oc.add_pixel(m, i, [E[i], nu[i]])
     oc.initialise()
     e = np.zeros((3, 3))
     e[0, 1] = e[1, 0] = 0.01
-    first_newton("c5_128_newton1", oc, np.tile(e.ravel(order="F"), (N, 1)), N, 3,
+    first_newton("c5_{}_newton1".format(n), oc, np.tile(e.ravel(order="F"), (N, 1)), N, 3,
                  {"res": np.array([n] * 3), "load": e})
 
 
@@ -171,4 +171,5 @@ if __name__ == "__main__":
     O.set_threads(int(os.environ.get("MO_THREADS", os.cpu_count() or 1)))
     which = sys.argv[1:] or ["c2", "c5", "c3"]
     for w in which:
-        {"c3": c3, "c2": c2, "c5": c5, "c3_243": lambda: c3(243), "c2_125": lambda: c2(125)}[w]()
+        {"c3": c3, "c2": c2, "c5": c5, "c3_243": lambda: c3(243), "c2_125": lambda: c2(125),
+         "c5_125": lambda: c5(125)}[w]()

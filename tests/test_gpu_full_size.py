@@ -9,7 +9,8 @@ the oracle's solve (tests/golden/make_solve_fixtures.py writes them):
   C2 128^3  small strain LE1, 40 periodic spheres, the shear load case
             (newton_cg to convergence)
   C5 128^3  C5's material (LE4, lognormal E, uniform nu per voxel) and load,
-            the first Newton iteration
+            the first Newton iteration, and the same at 125^3
+  C2 125^3  the C2 case on the odd grid (no even-grid policy)
 
 The full fields are too large for fixtures; each holds the iteration counts,
 field norms, homogenised stresses and the fields at 1024 sampled voxels.
@@ -135,18 +136,21 @@ def test_c3_243_first_newton_iteration(µ, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_c5_material_128_first_newton_iteration(µ, prec):
-    fx = _load("c5_128_newton1.npz")
-    n = 128
+@pytest.mark.parametrize("n", [128, 125])
+def test_c5_material_first_newton_iteration(µ, n, prec):
+    """C5's material at 128^3 and on the odd grid 125^3 (runtime-length
+    passes, no even-grid policy)."""
+    fx = _load("c5_{}_newton1.npz".format(n))
     N = n ** 3
     rng = np.random.default_rng(2026)
     E = np.exp(np.log(50e9) + 0.8 * rng.standard_normal(N))
     nu = rng.uniform(0.15, 0.45, N)
-    cell = µ.Cell([n] * 3, [1.0] * 3, µ.Formulation.small_strain, precision=prec, allow_even=True)
+    cell = µ.Cell([n] * 3, [1.0] * 3, µ.Formulation.small_strain, precision=prec,
+                  allow_even=n % 2 == 0)
     m = µ.material.MaterialLinearElastic4_3d.make(cell, "lognormal")
     m.add_pixels_by_index(np.arange(N, dtype=np.int64), np.stack([E, nu], axis=1))
-    _first_newton(µ, fx, cell, fx["load"].ravel(order="F"), prec, "c5 material 128^3",
-                  chaotic=True)
+    _first_newton(µ, fx, cell, fx["load"].ravel(order="F"), prec,
+                  "c5 material {}^3".format(n), chaotic=True)
 
 
 def _newton_case(µ, name, make_cell, prec):
