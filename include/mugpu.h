@@ -311,6 +311,10 @@ int mugpu_cg_graphs(int enable);
  * and alpha/beta; the outer loop recomputes the true residual b - A x with
  * the FP64 operator and corrects x in FP64.  Stops when
  * |b - A x|^2 <= tol^2 |b|^2 or after maxiter inner iterations in total.
+ * Each inner solve stops at inner_tol relative, or earlier once its
+ * residual, scaled back to b - A x, is tol |b| / 2 (the outer test passes
+ * with a factor of two to spare), so that the last inner solve of a solve
+ * whose tol is above inner_tol^2 goes no further than tol asks.
  * *iterations returns the inner CG iterations, *refinements the outer
  * FP64 residual evaluations.  Returns MUGPU_ECONVERGENCE when not
  * converged. */

@@ -1642,7 +1642,7 @@ int solve_mixed(mugpu_cell c, const MixedVecs& v, const void* d_rhs, void* d_x, 
                           FIN_STORE_RDR));
     MUGPU_TRY(sync_scalars(c->scal));
     const double rdr0 = g_host_scal[SC_RDR];
-    const double tol2i = inner_tol * inner_tol * rdr0;
+    const double tol2i = inner_threshold2(inner_tol, tol2, rr, rdr0);
     // SolverCG's loop, for (i = 0; its < maxiter && (r.r > tol2i || i == 0);
     // ++i, ++its), run on the device: iterations are queued kChunk at a time
     // ahead of the host, the FIN_BETA finalize applies the loop test and

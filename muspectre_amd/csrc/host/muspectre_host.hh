@@ -429,7 +429,8 @@ class SolverCG : public SolverBase {
   DeviceVector& solve(const DeviceVector& rhs) override;
   std::string get_name() const override { return "CG"; }
   // FP32 cells: relative tolerance of the inner FP32 CG between two FP64
-  // residual refinements
+  // residual refinements (an inner solve stops earlier once the outer
+  // tolerance is met with a factor of two to spare)
   void set_inner_tol(Real t) { inner_tol = t; }
   int64_t get_refinements() const { return refinements; }
 

@@ -38,6 +38,20 @@ inline int dev_alloc(T** p, size_t bytes, const char* what) {
   return dev_alloc(reinterpret_cast<void**>(p), bytes, what);
 }
 
+// Squared stopping threshold of one inner FP32 CG solve of the mixed-precision
+// refinement, for the inner system A d = r64 / |r64| whose start residual has
+// squared norm rdr: the inner solve stops at inner_tol relative or, when that
+// is more than the outer tolerance still asks for, once |r64| times its
+// relative residual is half the outer threshold (tol2 = tol^2 |b|^2, rr =
+// |r64|^2).  FP32 CG converges evenly to about 1e-4 relative and slows beyond
+// it, so the first inner solve stops at inner_tol; the last one of a solve at
+// a tolerance above inner_tol^2 stops as soon as the outer test will pass.
+inline double inner_threshold2(double inner_tol, double tol2, double rr, double rdr) {
+  const double need2 = 0.25 * tol2 / rr;
+  const double rel2 = inner_tol * inner_tol;
+  return (need2 > rel2 ? need2 : rel2) * rdr;
+}
+
 hipStream_t stream();
 // route the library's launches to another stream (nullptr: the library
 // stream); used by the serving loop, which drives two streams from one thread

@@ -656,7 +656,7 @@ int mugpu_server_s::advance(Request* r) {
       return residual_known(r);
     case ST_WAIT_RDR: {
       const double rdr = h[SC_RDR];
-      const double tol2i = cfg.inner_tol * cfg.inner_tol * rdr;
+      const double tol2i = inner_threshold2(cfg.inner_tol, r->tol2, r->rr, rdr);
       if (r->its >= r->maxiter) return after_inner(r, false);  // no inner iteration
       r->h[kHostTol2] = tol2i;
       r->inner_maxit = (int)(r->maxiter - r->its);
