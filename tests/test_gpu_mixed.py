@@ -149,3 +149,55 @@ def test_hep1_consistent_fp32_refined(µ, oracle, res):
     assert r.success and r.nb_it == o["nb_it"]
     assert _rel(_homog(r.stress, 9), _homog(o["stress"], 9)) < TOL
     assert _rel(_disp_grad(r.grad, 9, True), _disp_grad(o["grad"], 9, True)) < TOL_FIELD
+
+
+def test_inner_solves_stop_at_the_outer_tolerance(capi):
+    """An inner FP32 solve stops at inner_tol (1e-4) relative or, earlier,
+    once the outer test will pass with a factor of two to spare.  At tol 1e-8
+    the two inner solves both run to 1e-4.  At tol 1e-6 the second one needs
+    only about two of its four decades: the solve still meets its tolerance
+    with three FP64 residuals, in clearly fewer inner iterations (were every
+    inner solve run to 1e-4, the two solves would do the same work)."""
+    import ctypes
+    from muspectre_amd._device import DeviceBuffer
+    lib = capi.load()
+    res = [16, 24, 32]
+    N = int(np.prod(res))
+    n = N * 9
+    h = ctypes.c_void_p()
+    capi.check(lib.mugpu_cell_create(3, (ctypes.c_int * 3)(*res), (ctypes.c_double * 3)(1., 1., 1.),
+                                     capi.SMALL_STRAIN, capi.FP32, 1, ctypes.byref(h)))
+    try:
+        hard = np.random.default_rng(3).random(N) < 0.3
+        idx = np.arange(N, dtype=np.int64)
+        for E, sel in ((100e9, hard), (10e9, ~hard)):
+            m = ctypes.c_int()
+            capi.check(lib.mugpu_cell_add_material(h, capi.LE1, E, 0.3, 0.0, 0.0, ctypes.byref(m)))
+            pix = np.ascontiguousarray(idx[sel])
+            capi.check(lib.mugpu_material_add_pixels(
+                h, m.value, len(pix), pix.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), None))
+        capi.check(lib.mugpu_cell_initialise(h))
+        eps = np.zeros((3, 3))
+        eps[0, 1] = eps[1, 0] = 0.01
+        capi.check(lib.mugpu_cell_set_uniform_strain(h, eps.ravel(order="F").ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double))))
+        capi.check(lib.mugpu_cell_evaluate_stress_tangent(h))
+        P = ctypes.c_void_p()
+        capi.check(lib.mugpu_cell_stress(h, ctypes.byref(P)))
+        rhs = DeviceBuffer(8 * n)
+        capi.check(lib.mugpu_scale_copy(-1.0, P, rhs.ptr, n))
+        capi.check(lib.mugpu_cell_apply_projection(h, rhs.ptr))
+        out = {}
+        for tol in (1e-8, 1e-6):
+            x = DeviceBuffer(8 * n)
+            its, refs, rel = ctypes.c_int64(), ctypes.c_int(), ctypes.c_double()
+            capi.check(lib.mugpu_cg_solve_mixed(h, rhs.ptr, x.ptr, tol, 2000, 1e-4, ctypes.byref(its),
+                                                ctypes.byref(refs), ctypes.byref(rel)))
+            out[tol] = (its.value, refs.value, rel.value)
+    finally:
+        lib.mugpu_cell_destroy(h)
+    its8, refs8, rel8 = out[1e-8]
+    its6, refs6, rel6 = out[1e-6]
+    assert rel8 <= 1e-8 and rel6 <= 1e-6, out
+    assert refs6 == 3 and refs8 >= 3, out
+    assert its6 < 0.9 * its8, out
