@@ -32,6 +32,8 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <new>
+#include <thread>
 #include <vector>
 
 #include "mugpu_internal.hpp"
@@ -39,6 +41,34 @@
 namespace mugpu {
 double lame_lambda_h(double E, double nu) { return E * nu / ((1 + nu) * (1 - 2 * nu)); }
 double lame_mu_h(double E, double nu) { return E / (2 * (1 + nu)); }
+
+// lm[2v], lm[2v+1] = (lambda, mu) of voxel v.  Submission runs on the thread
+// that turns the serving loop, so a large request is split over up to eight
+// host threads, which also share the first-touch page faults of lm (left
+// uninitialised for that reason); every voxel's arithmetic is the same in
+// any split.  A thread that cannot be started leaves its range to this one.
+static void fill_lame(const double* E, const double* nu, int64_t N, double* lm) {
+  auto part = [=](int64_t v0, int64_t v1) {
+    for (int64_t v = v0; v < v1; ++v) {
+      lm[2 * v] = lame_lambda_h(E[v], nu[v]);
+      lm[2 * v + 1] = lame_mu_h(E[v], nu[v]);
+    }
+  };
+  const int T = (int)std::min<int64_t>(8, std::max<int64_t>(1, N / 65536));
+  std::vector<std::thread> pool;
+  int started = 1;
+  for (int t = 1; t < T; ++t) {
+    try {
+      pool.emplace_back(part, N * t / T, N * (t + 1) / T);
+    } catch (...) {
+      break;
+    }
+    ++started;
+  }
+  part(0, N / T);
+  for (int t = started; t < T; ++t) part(N * t / T, N * (t + 1) / T);
+  for (auto& th : pool) th.join();
+}
 }  // namespace mugpu
 
 using namespace mugpu;
@@ -207,7 +237,9 @@ struct Request {
   int dim = 0, nc = 0, res[3] = {1, 1, 1};
   double L[3] = {1, 1, 1};
   int64_t N = 0, nd = 0;
-  std::vector<double> lam_mu;
+  // per-voxel (lambda, mu), interleaved; uninitialised until submit fills
+  // it, freed once the first stage has completed
+  std::unique_ptr<double[]> lam_mu;
   double strain[9] = {0};
   int maxiter = 0;
   Layout lay{};
@@ -464,7 +496,7 @@ int mugpu_server_s::admit() {
     // cell set-up: per-voxel (lambda, mu), SC block, F = 0 + dE
     // (solvers.cc:96-145: set_uniform_strain(0), add_uniform_strain(dE))
     set_current_stream(sr);
-    MUGPU_HIP(hipMemcpyAsync(r->params, r->lam_mu.data(), sizeof(double) * 2 * r->N,
+    MUGPU_HIP(hipMemcpyAsync(r->params, r->lam_mu.get(), sizeof(double) * 2 * r->N,
                              hipMemcpyHostToDevice, sr));
     MUGPU_HIP(hipMemsetAsync(r->scal, 0, sizeof(double) * SC_COUNT, sr));
     const double zero[9] = {0};
@@ -630,7 +662,7 @@ int mugpu_server_s::advance(Request* r) {
   const double tol = cfg.cg_tol;
   switch (r->st) {
     case ST_WAIT_EVAL: {
-      if (!r->lam_mu.empty()) std::vector<double>().swap(r->lam_mu);
+      r->lam_mu.reset();
       r->stress_norm = std::sqrt(h[SC_TMP0]);
       if (r->test(cfg.newton_tol, cfg.equil_tol)) return finish(r);
       // SolverCG::solve, mixed precision: x = 0, |b|^2, then refinement
@@ -882,11 +914,13 @@ extern "C" int mugpu_server_submit(mugpu_server s, int dim, const int* res, cons
     r->N *= res[d];
   }
   r->nd = r->N * r->nc;
-  r->lam_mu.resize((size_t)(2 * r->N));
-  for (int64_t v = 0; v < r->N; ++v) {
-    r->lam_mu[(size_t)(2 * v)] = lame_lambda_h(young[v], poisson[v]);
-    r->lam_mu[(size_t)(2 * v + 1)] = lame_mu_h(young[v], poisson[v]);
+  r->lam_mu.reset(new (std::nothrow) double[(size_t)(2 * r->N)]);
+  if (!r->lam_mu) {
+    set_error("host allocation of a request's material parameters (%lld voxels) failed",
+              (long long)r->N);
+    return MUGPU_EMEMORY;
   }
+  fill_lame(young, poisson, r->N, r->lam_mu.get());
   for (int c = 0; c < r->nc; ++c) r->strain[c] = strain[c];
   r->maxiter = cg_maxiter > 0 ? cg_maxiter : s->cfg.cg_maxiter;
   r->lay = layout_of(dim, res);

@@ -80,8 +80,12 @@ def _same(a, b):
     np.testing.assert_array_equal(a["mean_stress"], b["mean_stress"])
 
 
-@pytest.mark.parametrize("res", [[64, 64], [32, 32, 32], [48, 16, 32]], ids=lambda r: "x".join(map(str, r)))
+@pytest.mark.parametrize("res", [[64, 64], [32, 32, 32], [48, 16, 32], [96, 64, 32], [512, 256]],
+                         ids=lambda r: "x".join(map(str, r)))
 def test_request_equals_cell_newton_cg(µ, serving, res):
+    """A served request computes exactly what newton_cg computes on a Cell;
+    96x64x32 and 512x256 are large enough for submit to split the
+    per-voxel Lame parameters over three and two host threads."""
     q = _request(res, 3)
     out, _ = _serve(serving, [q])
     r = out[0]
