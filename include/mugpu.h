@@ -378,9 +378,12 @@ int mugpu_server_destroy(mugpu_server server);
 int mugpu_server_footprint(int dim, const int *resolutions, size_t *bytes);
 /* queue one solve; young/poisson per voxel (row-major pixels), strain the
  * symmetric dim x dim macroscopic strain (column-major); cg_maxiter 0 takes
- * the server's.  MUGPU_ESOLVER for an asymmetric strain, MUGPU_EMEMORY when
- * the footprint exceeds the whole budget, MUGPU_EARG for a resolution
- * without compiled passes. */
+ * the server's.  The per-voxel parameters are copied (as Lame pairs) before
+ * the call returns; for requests of 131,072 voxels or more the copy runs on
+ * up to eight short-lived host threads.  MUGPU_ESOLVER for an asymmetric
+ * strain, MUGPU_EMEMORY when the footprint exceeds the whole budget or the
+ * host copy cannot be allocated, MUGPU_EARG for a resolution without
+ * compiled passes. */
 int mugpu_server_submit(mugpu_server server, int dim, const int *resolutions,
                         const double *lengths, const double *young,
                         const double *poisson, const double *strain,
