@@ -34,7 +34,8 @@ Workloads (synthetic inputs generated from fixed seeds):
       FP32 + refined CG to 1e-6.  Phase 1 queues every request at t = 0
       (saturated throughput, RVE solves/s); phase 2 replays a fresh stream
       with Poisson arrivals at --load times that throughput (p50/p99
-      latency from each request's arrival time).
+      latency from each request's arrival time).  Each phase's per-voxel
+      E and nu arrays are formed in host memory before its timed region.
 """
 import argparse
 import ctypes
@@ -549,10 +550,16 @@ def c4_stream(n, seed0):
         return list(ex.map(c4_request, range(seed0, seed0 + n), chunksize=16))
 
 
+def c4_inputs(reqs):
+    """The requests' per-voxel E and nu as host arrays: the inputs a client
+    hands the server, formed before a timed phase starts."""
+    return [c4_arrays(q) for q in reqs]
+
+
 def c4_run(srv, reqs, arrivals):
-    """Submit each request at its arrival time (seconds after the start) and
-    turn the loop until all have finished; returns (wall seconds, latencies,
-    results)."""
+    """Submit each request (res, E, nu, strain: host arrays formed before the
+    call) at its arrival time (seconds after the start) and turn the loop
+    until all have finished; returns (wall seconds, latencies, results)."""
     n = len(reqs)
     order = np.argsort(arrivals, kind="stable")
     t_done = np.zeros(n)
@@ -565,7 +572,7 @@ def c4_run(srv, reqs, arrivals):
         now = time.perf_counter() - t0
         while nxt < n and arrivals[order[nxt]] <= now:
             i = order[nxt]
-            res, E, nu, eps = c4_arrays(reqs[i])
+            res, E, nu, eps = reqs[i]
             idmap[srv.submit(res, E, nu, eps)] = i
             nxt += 1
         srv.progress(wait=False)
@@ -595,8 +602,11 @@ def c4_main(args):
     warm = {}
     for q in sat_reqs + poi_reqs:
         warm.setdefault(tuple(q[0]), q)
-    c4_run(srv, list(warm.values()), np.zeros(len(warm)))
-    wall, lat, res = c4_run(srv, sat_reqs, np.zeros(n))
+    c4_run(srv, c4_inputs(warm.values()), np.zeros(len(warm)))
+    print("bench: c4 forming the saturated phase's inputs", file=sys.stderr, flush=True)
+    inputs = c4_inputs(sat_reqs)
+    wall, lat, res = c4_run(srv, inputs, np.zeros(n))
+    del inputs
     throughput = n / wall
     voxel_iters = sum(int(np.prod(sat_reqs[i][0])) * r["cg_iterations"] for i, r in res.items())
     failed = sum(1 for r in res.values() if not r["ok"])
@@ -604,7 +614,10 @@ def c4_main(args):
           file=sys.stderr, flush=True)
     rate = args.load * throughput
     arr = np.cumsum(np.random.default_rng(7).exponential(1.0 / rate, n))
-    wall2, lat2, res2 = c4_run(srv, poi_reqs, arr)
+    print("bench: c4 forming the Poisson phase's inputs", file=sys.stderr, flush=True)
+    inputs = c4_inputs(poi_reqs)
+    wall2, lat2, res2 = c4_run(srv, inputs, arr)
+    del inputs
     st = srv.stats()
     srv.close()
     p50, p99 = np.percentile(lat2, [50, 99])
@@ -633,7 +646,8 @@ def c4_main(args):
         "data": "synthetic",
         "config": {"workload": "c4", "requests": n, "mix": "3D p=.7 {32,48,64,96}^3, 2D {64..512}^2",
                    "cg_tol": 1e-6, "formulation": "small", "max_batch": 32, "iters_per_launch": 8,
-                   "graphs": not args.no_graphs},
+                   "graphs": not args.no_graphs,
+                   "inputs": "per-voxel E and nu host arrays, formed before each timed phase"},
         "saturated": {"requests": n, "wall_s": round(wall, 3), "solves_per_s": round(throughput, 2),
                       "voxel_cg_iters_per_s": round(voxel_iters / wall, 1), "failed": failed},
         "poisson": {"rate_per_s": round(rate, 2), "load": args.load, "requests": n,
